@@ -1,0 +1,200 @@
+/*
+ * trust.h -- C ABI of the MI355X (gfx950) TRUST triangle-counting library.
+ *
+ * Plain C types only: pointers, sizes and fixed-width integers.  Every entry
+ * point returns 0 (TRUST_OK) or a negative TRUST_ERR_* code and never exits
+ * the process; trust_last_error() gives the message of the calling thread's
+ * last failure.  Citations name the reference interface each entry point
+ * replaces (paths relative to the wzbxpy/TRUST repository root).
+ */
+#ifndef TRUST_H
+#define TRUST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define TRUST_OK 0
+#define TRUST_ERR_ARG (-1)      /* invalid argument */
+#define TRUST_ERR_IO (-2)       /* missing / unreadable / unwritable file */
+#define TRUST_ERR_FORMAT (-3)   /* malformed CSR (sizes, offsets, ids) */
+#define TRUST_ERR_OOM (-4)      /* host or device allocation failed */
+#define TRUST_ERR_HIP (-5)      /* HIP runtime error */
+#define TRUST_ERR_OVERFLOW (-6) /* a size exceeds what the representation holds */
+#define TRUST_ERR_NODEV (-7)    /* no usable GPU */
+
+/* ---- schedules and flags ---------------------------------------------- */
+#define TRUST_SCHED_DEGREE 0  /* plain degree-threshold split (reference USE_CTA) */
+#define TRUST_SCHED_LEARNED 1 /* MLP cost model: tiers + longest-first queue */
+
+#define TRUST_FLAG_FORCE_SPILL 1u  /* every hashed vertex uses an HBM table */
+#define TRUST_FLAG_OFFSETS64 2u    /* keep int64 offsets on the device even if E < 2^31 */
+#define TRUST_FLAG_NO_VALIDATE 4u  /* skip the host-side CSR validation */
+#define TRUST_FLAG_WAVE_ONLY 8u    /* no workgroup tier: every vertex that fits a wave table uses one wave */
+
+/*
+ * Launch and schedule parameters.  threads / blocks / chunk / rank /
+ * total_process carry the meaning of the reference's argv[3], argv[4],
+ * argv[5], the MPI rank and argv[2] (Without-graph-partition/mpi_main.cpp:78-81).
+ * Zero means "choose automatically" for threads, blocks, chunk, wave_slots
+ * and lds_budget.  Fill with trust_params_init() before changing fields.
+ */
+typedef struct trust_params {
+    int32_t threads;       /* work-items per workgroup: multiple of 64, <= 1024 */
+    int32_t blocks;        /* persistent workgroups in the launch */
+    int32_t chunk;         /* vertices taken per queue grab */
+    int32_t rank;          /* logical rank of this call, 0 <= rank < total_process */
+    int32_t total_process; /* number of logical ranks the work is split over */
+    int32_t schedule;      /* TRUST_SCHED_* */
+    int32_t hub_threshold; /* walk degree above which a whole workgroup takes a vertex (reference: 100) */
+    int32_t wave_slots;    /* per-wave LDS hash-table slots (power of two) */
+    int32_t lds_budget;    /* bytes of LDS per workgroup for the hub table */
+    uint32_t flags;        /* TRUST_FLAG_* */
+    int32_t device;        /* HIP device index */
+    int32_t reserved[5];
+} trust_params;
+
+/* Result of one counting call. */
+typedef struct trust_result {
+    int64_t vertices;      /* V (rows of the CSR) */
+    int64_t edges;         /* E (entries of adjacent.bin = undirected edges) */
+    uint64_t triangles;    /* exact count (this rank's share) */
+    uint64_t probes;       /* W: hash lookups performed by this call */
+    double seconds;        /* device time, planning + counting (hipEvents) */
+    double count_seconds;  /* device time of the counting launch alone */
+    double plan_seconds;   /* device time of the planning kernels */
+    double h2d_seconds;    /* host-to-device copy time (not part of seconds) */
+    double gteps;          /* edges / seconds / 1e9 */
+    double alg_bytes;      /* algorithmic HBM bytes of the counting launch */
+    int64_t n_spill;       /* vertices counted with an HBM hash table */
+    int64_t n_hub;         /* vertices counted by a whole workgroup (LDS table) */
+    int64_t n_wave;        /* vertices counted by one wavefront (LDS table) */
+    int32_t threads;       /* launch shape actually used */
+    int32_t blocks;
+    int32_t hub_slots;     /* LDS hub-table slots per workgroup */
+    int32_t wave_slots;    /* LDS table slots per wavefront */
+} trust_result;
+
+/* Library-allocated CSR (trust_csr_free releases it). */
+typedef struct trust_csr {
+    int64_t vertices;
+    int64_t edges;
+    int64_t *beg; /* vertices+1 offsets (begin.bin layout) */
+    int32_t *adj; /* edges ids (adjacent.bin layout) */
+} trust_csr;
+
+/* Drop-in result of Triangle_count (Without-graph-partition/TC.cuh:2-8), widened. */
+typedef struct trust_arguments {
+    int64_t edge_count; /* reference: int */
+    uint64_t count;     /* reference: long long */
+    double time;        /* seconds of the counting launch */
+    int64_t degree;     /* edge_count / vertices */
+    int64_t vertices;   /* reference: int */
+} trust_arguments;
+
+typedef struct trust_ctx trust_ctx;
+
+/* ---- library ---------------------------------------------------------- */
+const char *trust_last_error(void);
+const char *trust_version(void);
+/* Defaults: threads 0, blocks 0, chunk 0, rank 0, total_process 1,
+ * schedule DEGREE, hub_threshold 100, wave_slots 0, lds_budget 0. */
+void trust_params_init(trust_params *p);
+/* Number of HIP devices (0 when none); negative on runtime failure. */
+int trust_device_count(void);
+
+/* ---- drop-in counting entry points ------------------------------------ */
+/*
+ * Replaces Triangle_count(rank, folder, args, total_process, threads, blocks,
+ * chunk_size) of Without-graph-partition/TC.cuh:10 (TC.cu:602-731): loads
+ * <folder>/begin.bin and <folder>/adjacent.bin, counts this rank's share of
+ * vertex chunks (chunk c is rank c % total_process, TC.cu:259,433) and fills
+ * *out.  threads/blocks/chunk_size <= 0 select automatic values.
+ */
+int trust_triangle_count(int rank, const char *folder, int total_process, int threads,
+                         int blocks, int chunk_size, trust_arguments *out);
+
+/*
+ * Replaces Triangle_count(..., partition_num) of With-graph-partition/TC.cuh:10
+ * (TC.cu:976-1157): rank selects the tile job (i,j,k) = (rank%n, rank/n%n,
+ * rank/n/n%n) over <folder>/partition{i}_{j}/ and rank/n^3 of total_process/n^3
+ * selects the chunk share inside that job (TC.cu:985-994,1024-1025).
+ */
+int trust_triangle_count_partition(int rank, const char *folder, int total_process,
+                                   int threads, int blocks, int chunk_size,
+                                   int partition_num, trust_arguments *out);
+
+/* Count over a host CSR (copied to the device inside the call). */
+int trust_count_csr(const int64_t *beg, const int32_t *adj, int64_t vertices, int64_t edges,
+                    const trust_params *p, trust_result *r);
+
+/*
+ * Sum over all n^3 (i,j,k) jobs of a 2D hash partition held in host memory,
+ * streamed through one GPU with asynchronous tile copies.  tiles[i*n+j] is
+ * tile (i,j) in the layout of Preprocess/partition.cpp:72-121.
+ */
+int trust_count_tiles(const trust_csr *tiles, int n, const trust_params *p, trust_result *r);
+
+/* ---- resident graphs -------------------------------------------------- */
+int trust_ctx_create(int device, trust_ctx **out);
+void trust_ctx_destroy(trust_ctx *ctx);
+/* Copy a CSR to the device once; later counts reuse it.  h2d_seconds nullable. */
+int trust_ctx_load_csr(trust_ctx *ctx, const int64_t *beg, const int32_t *adj,
+                       int64_t vertices, int64_t edges, uint32_t flags, double *h2d_seconds);
+int trust_ctx_count(trust_ctx *ctx, const trust_params *p, trust_result *r);
+/*
+ * Instrumented count: per-vertex cycle counts (clock64 around each vertex's
+ * work) and probe counts.  cycles/probes have `vertices` entries (nullable).
+ */
+int trust_ctx_profile_vertices(trust_ctx *ctx, const trust_params *p, uint64_t *cycles,
+                               uint64_t *probes, trust_result *r);
+
+/* ---- input contract: files, preprocessing, tiles ---------------------- */
+/* Read <folder>/begin.bin + adjacent.bin (Without-graph-partition/graph.hpp:18-67)
+ * with size and content validation. */
+int trust_csr_load(const char *folder, trust_csr *out);
+int trust_csr_save(const char *folder, const trust_csr *csr);
+/* Check offsets (monotone, beg[0]=0, beg[V]=E) and ids (< V). */
+int trust_csr_validate(const int64_t *beg, const int32_t *adj, int64_t vertices, int64_t edges);
+void trust_csr_free(trust_csr *csr);
+
+/*
+ * Edge list -> degree-oriented CSR, byte-identical to running
+ * Preprocess/fromDirectToUndirect.cpp then Preprocess/preprocess_outdegree.cpp
+ * on the same list: ids compacted in ascending order, undirected,
+ * deduplicated, oriented from lower to higher position of an ascending
+ * degree sort, rows ordered by out-degree classes (>100, 2..100, <2) with a
+ * first-touch relabel inside each class.  src/dst hold m pairs of
+ * non-negative ids.
+ */
+int trust_preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, trust_csr *out);
+
+/* Tile (i,j) of an n x n hash partition (Preprocess/partition.cpp:72-121). */
+int trust_partition_csr(const trust_csr *csr, int n, trust_csr *tiles /* n*n */);
+/* Write partition{i}_{j}/{begin.bin,adjacent.bin,map,division} under folder
+ * (Preprocess/partition.cpp:101-165). */
+int trust_partition_write(const char *folder, const trust_csr *csr, int n);
+
+/* ---- seeded synthetic generators (edge lists, not yet preprocessed) --- */
+/* R-MAT: m = edge_factor << scale samples into src/dst (caller-allocated). */
+int trust_gen_rmat(int scale, int64_t m, double a, double b, double c, uint64_t seed,
+                   int32_t *src, int32_t *dst);
+/* Chung-Lu: m samples, weights (i+i0)^(-1/(gamma-1)) scaled so the largest
+ * expected degree is max_degree. */
+int trust_gen_chung_lu(int64_t n, int64_t m, double gamma, double max_degree, uint64_t seed,
+                       int32_t *src, int32_t *dst);
+/* Erdos-Renyi G(n, m): m uniform pairs. */
+int trust_gen_erdos_renyi(int64_t n, int64_t m, uint64_t seed, int32_t *src, int32_t *dst);
+/* Barabasi-Albert: n vertices, k edges per new vertex; returns k*(n-k) pairs
+ * (caller allocates k*n). *m_out receives the pair count. */
+int trust_gen_barabasi_albert(int64_t n, int k, uint64_t seed, int32_t *src, int32_t *dst,
+                              int64_t *m_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRUST_H */

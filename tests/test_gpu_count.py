@@ -1,0 +1,182 @@
+"""GPU parity: the HIP counting path against the CPU oracle (bit-exact uint64).
+
+Every count here goes through the C ABI into the gfx950 kernels; the oracle
+is the checker only.  The count must not depend on any launch or schedule
+parameter, so the sweeps below double as invariance tests of the hash-table
+sizing, the tier split, the queues and the logical-rank split.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import trust_amd as T
+from conftest import CLOSED_FORMS, closed_form_csr, golden_cases, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rmat16():
+    # R-MAT scale 16, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05), seed 1
+    beg, adj = T.rmat_csr(16, 16, seed=1)
+    want = oracle.count_hash(beg, adj)
+    assert want == oracle.count_merge(beg, adj)
+    return beg, adj, want, oracle.probes(beg, adj)
+
+
+@pytest.fixture(scope="module")
+def ctx16(gpu, rmat16):
+    beg, adj, want, w = rmat16
+    c = T.Context(gpu)
+    c.load(beg, adj)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name", sorted(CLOSED_FORMS))
+def test_closed_forms(gpu, name):
+    beg, adj, want = closed_form_csr(name)
+    print(f"[gpu] closed form {name}", flush=True)
+    assert T.count_csr(beg, adj, device=gpu).triangles == want
+    assert T.count_csr(beg, adj, device=gpu, flags=T.FLAG_FORCE_SPILL).triangles == want
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_golden(gpu, name):
+    beg, adj, meta = load_golden(name)
+    print(f"[gpu] golden {name}", flush=True)
+    r = T.count_csr(beg, adj, device=gpu)
+    assert r.triangles == meta["triangles_networkx"]
+    assert r.probes == oracle.probes(beg, adj)
+
+
+def test_rmat16_default(ctx16, rmat16):
+    beg, adj, want, w = rmat16
+    r = ctx16.count()
+    print(f"[gpu] rmat16 tiers {r.n_spill}/{r.n_hub}/{r.n_wave} t={r.seconds:.6f}s", flush=True)
+    assert r.triangles == want
+    assert r.probes == w
+    assert r.n_hub > 0 and r.n_wave > 0
+    assert r.n_hub + r.n_wave + r.n_spill == int((np.diff(beg) >= 2).sum())
+
+
+@pytest.mark.parametrize("threads", [64, 128, 256, 512, 1024])
+def test_rmat16_threads(ctx16, rmat16, threads):
+    assert ctx16.count(threads=threads).triangles == rmat16[2]
+
+
+@pytest.mark.parametrize("kw", [
+    dict(threads=1024, blocks=1024, chunk=1),       # the reference README launch
+    dict(blocks=1), dict(blocks=7, chunk=3), dict(chunk=64),
+    dict(hub_threshold=0), dict(hub_threshold=32), dict(hub_threshold=1 << 30),
+    dict(wave_slots=64, threads=256), dict(wave_slots=2048, threads=256),
+    dict(lds_budget=4096, threads=64, wave_slots=256),
+    dict(flags=T.FLAG_FORCE_SPILL), dict(flags=T.FLAG_WAVE_ONLY),
+    dict(flags=T.FLAG_FORCE_SPILL, threads=64, blocks=3),
+])
+def test_rmat16_invariance(ctx16, rmat16, kw):
+    print(f"[gpu] rmat16 {kw}", flush=True)
+    r = ctx16.count(**kw)
+    assert r.triangles == rmat16[2]
+    assert r.probes == rmat16[3]
+
+
+def test_rmat16_offsets64(gpu, rmat16):
+    beg, adj, want, w = rmat16
+    with T.Context(gpu) as c:
+        c.load(beg, adj, flags=T.FLAG_OFFSETS64)
+        assert c.count().triangles == want
+        assert c.count(flags=T.FLAG_FORCE_SPILL).triangles == want
+
+
+@pytest.mark.parametrize("P", [2, 3, 8])
+def test_logical_ranks_sum(ctx16, rmat16, P):
+    parts = [ctx16.count(rank=r, total_process=P, chunk=2).triangles for r in range(P)]
+    assert sum(parts) == rmat16[2]
+    assert min(parts) > 0
+
+
+def test_repeat_is_deterministic(ctx16, rmat16):
+    counts = {ctx16.count().triangles for _ in range(5)}
+    assert counts == {rmat16[2]}
+
+
+def test_instrumented_probes_match_oracle(ctx16, rmat16):
+    beg, adj, want, w = rmat16
+    cyc, prb, r = ctx16.profile_vertices()
+    assert r.triangles == want
+    _, per = oracle.probes(beg, adj, per_vertex=True)
+    assert np.array_equal(prb.astype(np.int64), per)
+    d = np.diff(beg)
+    assert np.all(cyc[d >= 2] > 0) and np.all(cyc[d < 2] == 0)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_dense_graphs(gpu, seed):
+    # dense random graphs: many hub-tier vertices and long probe lists
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(300, 900))
+    m = int(rng.integers(20 * n, 60 * n))
+    src = rng.integers(0, n, m).astype(np.int32)
+    dst = rng.integers(0, n, m).astype(np.int32)
+    beg, adj = T.preprocess_edges(src, dst)
+    want = oracle.count_hash(beg, adj)
+    for kw in ({}, dict(flags=T.FLAG_FORCE_SPILL), dict(threads=64), dict(hub_threshold=8)):
+        assert T.count_csr(beg, adj, device=gpu, **kw).triangles == want
+
+
+def test_big_hub_spills_to_hbm(gpu):
+    # K_400 has d+ up to 399: with a 4 KB LDS budget the biggest tables do
+    # not fit and take the HBM tier without being asked to.
+    beg, adj, _ = closed_form_csr("K150")
+    e = np.array([(i, j) for i in range(400) for j in range(i + 1, 400)], np.int32)
+    beg, adj = T.preprocess_edges(e[:, 0], e[:, 1])
+    r = T.count_csr(beg, adj, device=gpu, lds_budget=4096, threads=64, wave_slots=64)
+    assert r.triangles == 400 * 399 * 398 // 6
+    assert r.n_spill > 0
+
+
+def test_unsorted_general_csr(gpu):
+    # any valid oriented CSR, not only the reference's class order: reverse
+    # the vertex order and shuffle every list
+    beg, adj = T.rmat_csr(12, 16, seed=5)
+    want = oracle.count_hash(beg, adj)
+    V = len(beg) - 1
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(V)            # new id of old vertex
+    inv = np.argsort(perm)
+    d = np.diff(beg)
+    nb = np.zeros(V + 1, np.int64)
+    nb[1:] = np.cumsum(d[inv])
+    na = np.empty_like(adj)
+    for new in range(V):
+        old = inv[new]
+        lst = perm[adj[beg[old]:beg[old + 1]]]
+        rng.shuffle(lst)
+        na[nb[new]:nb[new + 1]] = lst
+    assert T.count_csr(nb, na, device=gpu).triangles == want
+
+
+def test_degenerate_graphs(gpu):
+    for beg, adj in (([0], []), ([0, 0], []), ([0, 1, 1], [1]), ([0, 2, 3, 3], [1, 2, 2])):
+        r = T.count_csr(np.array(beg), np.array(adj), device=gpu)
+        want = 1 if len(adj) == 3 else 0
+        assert r.triangles == want
+
+
+def test_drop_in_entry_point_and_cli(gpu, tmp_path, rmat16):
+    beg, adj, want, _ = rmat16
+    folder = str(tmp_path)
+    T.save_folder(folder, beg, adj)
+    a = T.triangle_count(0, folder, 1, 1024, 1024, 1)
+    assert a.count == want and a.edge_count == len(adj) and a.vertices == len(beg) - 1
+    exe = os.path.join(T.BIN_DIR, "trianglecounting")
+    out = subprocess.run([exe, folder, "3", "1024", "1024", "1"], capture_output=True, text=True,
+                         check=True, timeout=120).stdout
+    fields = out.strip().split(",")
+    assert fields[0] == folder
+    assert int(fields[1]) == len(beg) - 1 and int(fields[2]) == len(adj)
+    assert int(fields[3]) == want

@@ -1,0 +1,351 @@
+"""Python binding of libtrust_amd.so (the MI355X TRUST triangle-counting library).
+
+Thin ctypes layer over the C ABI in include/trust.h, used by the tests and
+bench.py.  The library is loaded from this package directory; when it is
+missing every call raises -- there is no Python or CPU fallback for any
+counting path.
+
+Reference interfaces mirrored here (paths relative to the wzbxpy/TRUST root):
+  triangle_count            Triangle_count, Without-graph-partition/TC.cuh:10
+  triangle_count_partition  Triangle_count, With-graph-partition/TC.cuh:10
+  preprocess_edges          Preprocess/fromDirectToUndirect.cpp + preprocess_outdegree.cpp
+  partition_csr / write     Preprocess/partition.cpp
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtrust_amd.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+TRUST_OK = 0
+ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2, -3, -4, -5, -6, -7)
+SCHED_DEGREE, SCHED_LEARNED = 0, 1
+FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY = 1, 2, 4, 8
+
+
+class TrustError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"trust error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("threads", C.c_int32), ("blocks", C.c_int32), ("chunk", C.c_int32),
+        ("rank", C.c_int32), ("total_process", C.c_int32), ("schedule", C.c_int32),
+        ("hub_threshold", C.c_int32), ("wave_slots", C.c_int32), ("lds_budget", C.c_int32),
+        ("flags", C.c_uint32), ("device", C.c_int32), ("reserved", C.c_int32 * 5),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("vertices", C.c_int64), ("edges", C.c_int64), ("triangles", C.c_uint64),
+        ("probes", C.c_uint64), ("seconds", C.c_double), ("count_seconds", C.c_double),
+        ("plan_seconds", C.c_double), ("h2d_seconds", C.c_double), ("gteps", C.c_double),
+        ("alg_bytes", C.c_double), ("n_spill", C.c_int64), ("n_hub", C.c_int64),
+        ("n_wave", C.c_int64), ("threads", C.c_int32), ("blocks", C.c_int32),
+        ("hub_slots", C.c_int32), ("wave_slots", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Csr(C.Structure):
+    _fields_ = [("vertices", C.c_int64), ("edges", C.c_int64),
+                ("beg", C.POINTER(C.c_int64)), ("adj", C.POINTER(C.c_int32))]
+
+
+class Arguments(C.Structure):
+    _fields_ = [("edge_count", C.c_int64), ("count", C.c_uint64), ("time", C.c_double),
+                ("degree", C.c_int64), ("vertices", C.c_int64)]
+
+
+_P64, _P32 = C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+_PU64 = C.POINTER(C.c_uint64)
+
+# name -> (restype, argtypes); every symbol of include/trust.h
+SIGNATURES = {
+    "trust_last_error": (C.c_char_p, []),
+    "trust_version": (C.c_char_p, []),
+    "trust_params_init": (None, [C.POINTER(Params)]),
+    "trust_device_count": (C.c_int, []),
+    "trust_triangle_count": (C.c_int, [C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(Arguments)]),
+    "trust_triangle_count_partition": (C.c_int, [C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int,
+                                                 C.c_int, C.c_int, C.POINTER(Arguments)]),
+    "trust_count_csr": (C.c_int, [_P64, _P32, C.c_int64, C.c_int64, C.POINTER(Params),
+                                  C.POINTER(Result)]),
+    "trust_count_tiles": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Params),
+                                    C.POINTER(Result)]),
+    "trust_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "trust_ctx_destroy": (None, [C.c_void_p]),
+    "trust_ctx_load_csr": (C.c_int, [C.c_void_p, _P64, _P32, C.c_int64, C.c_int64, C.c_uint32,
+                                     C.POINTER(C.c_double)]),
+    "trust_ctx_count": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Result)]),
+    "trust_ctx_profile_vertices": (C.c_int, [C.c_void_p, C.POINTER(Params), _PU64, _PU64,
+                                             C.POINTER(Result)]),
+    "trust_csr_load": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
+    "trust_csr_save": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
+    "trust_csr_validate": (C.c_int, [_P64, _P32, C.c_int64, C.c_int64]),
+    "trust_csr_free": (None, [C.POINTER(Csr)]),
+    "trust_preprocess_edges": (C.c_int, [_P32, _P32, C.c_int64, C.POINTER(Csr)]),
+    "trust_partition_csr": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Csr)]),
+    "trust_partition_write": (C.c_int, [C.c_char_p, C.POINTER(Csr), C.c_int]),
+    "trust_gen_rmat": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                 C.c_uint64, _P32, _P32]),
+    "trust_gen_chung_lu": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_uint64,
+                                     _P32, _P32]),
+    "trust_gen_erdos_renyi": (C.c_int, [C.c_int64, C.c_int64, C.c_uint64, _P32, _P32]),
+    "trust_gen_barabasi_albert": (C.c_int, [C.c_int64, C.c_int, C.c_uint64, _P32, _P32,
+                                            C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libtrust_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _check(rc: int):
+    if rc != TRUST_OK:
+        raise TrustError(rc, lib().trust_last_error().decode())
+
+
+def _i64(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(_P64)
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_P32)
+
+
+def device_count() -> int:
+    n = lib().trust_device_count()
+    if n < 0:
+        _check(n)
+    return n
+
+
+def make_params(**kw) -> Params:
+    p = Params()
+    lib().trust_params_init(C.byref(p))
+    for k, v in kw.items():
+        if k == "reserved":
+            for i, x in enumerate(v):
+                p.reserved[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+# ---------------------------------------------------------------- CSR helpers
+def _take_csr(c: Csr):
+    try:
+        V, E = c.vertices, c.edges
+        beg = np.ctypeslib.as_array(c.beg, shape=(V + 1,)).copy()
+        adj = (np.ctypeslib.as_array(c.adj, shape=(E,)).copy() if E > 0
+               else np.zeros(0, np.int32))
+    finally:
+        lib().trust_csr_free(C.byref(c))
+    return beg, adj
+
+
+def _as_csr(beg, adj):
+    beg, pb = _i64(beg)
+    adj, pa = _i32(adj)
+    return Csr(len(beg) - 1, len(adj), pb, pa), (beg, adj)
+
+
+def load_folder(folder: str):
+    c = Csr()
+    _check(lib().trust_csr_load(folder.encode(), C.byref(c)))
+    return _take_csr(c)
+
+
+def save_folder(folder: str, beg, adj):
+    c, keep = _as_csr(beg, adj)
+    _check(lib().trust_csr_save(folder.encode(), C.byref(c)))
+
+
+def validate(beg, adj):
+    beg, pb = _i64(beg)
+    adj, pa = _i32(adj)
+    _check(lib().trust_csr_validate(pb, pa, len(beg) - 1, len(adj)))
+
+
+def preprocess_edges(src, dst):
+    """Raw edge pairs -> (beg int64, adj int32) in the reference's oriented layout."""
+    src, ps = _i32(src)
+    dst, pd = _i32(dst)
+    c = Csr()
+    _check(lib().trust_preprocess_edges(ps, pd, len(src), C.byref(c)))
+    return _take_csr(c)
+
+
+def partition_csr(beg, adj, n: int):
+    """n x n hash-partition tiles; returns list of (beg, adj), index i*n+j."""
+    c, keep = _as_csr(beg, adj)
+    tiles = (Csr * (n * n))()
+    _check(lib().trust_partition_csr(C.byref(c), n, tiles))
+    return [_take_csr(tiles[k]) for k in range(n * n)]
+
+
+def partition_write(folder: str, beg, adj, n: int):
+    c, keep = _as_csr(beg, adj)
+    _check(lib().trust_partition_write(folder.encode(), C.byref(c), n))
+
+
+# ---------------------------------------------------------------- generators
+RMAT_ABC = (0.57, 0.19, 0.19)
+
+
+def gen_rmat(scale: int, edge_factor: int = 16, seed: int = 1, abc=RMAT_ABC):
+    m = edge_factor << scale
+    src = np.empty(m, np.int32)
+    dst = np.empty(m, np.int32)
+    _check(lib().trust_gen_rmat(scale, m, abc[0], abc[1], abc[2], seed,
+                                src.ctypes.data_as(_P32), dst.ctypes.data_as(_P32)))
+    return src, dst
+
+
+def gen_chung_lu(n: int, m: int, gamma: float = 2.5, max_degree: float = 800.0, seed: int = 1):
+    src = np.empty(m, np.int32)
+    dst = np.empty(m, np.int32)
+    _check(lib().trust_gen_chung_lu(n, m, gamma, max_degree, seed,
+                                    src.ctypes.data_as(_P32), dst.ctypes.data_as(_P32)))
+    return src, dst
+
+
+def gen_erdos_renyi(n: int, m: int, seed: int = 1):
+    src = np.empty(m, np.int32)
+    dst = np.empty(m, np.int32)
+    _check(lib().trust_gen_erdos_renyi(n, m, seed, src.ctypes.data_as(_P32),
+                                       dst.ctypes.data_as(_P32)))
+    return src, dst
+
+
+def gen_barabasi_albert(n: int, k: int, seed: int = 1):
+    src = np.empty(n * k, np.int32)
+    dst = np.empty(n * k, np.int32)
+    m = C.c_int64(0)
+    _check(lib().trust_gen_barabasi_albert(n, k, seed, src.ctypes.data_as(_P32),
+                                           dst.ctypes.data_as(_P32), C.byref(m)))
+    return src[: m.value], dst[: m.value]
+
+
+def rmat_csr(scale: int, edge_factor: int = 16, seed: int = 1):
+    return preprocess_edges(*gen_rmat(scale, edge_factor, seed))
+
+
+# ---------------------------------------------------------------- counting
+def count_csr(beg, adj, **params) -> Result:
+    beg, pb = _i64(beg)
+    adj, pa = _i32(adj)
+    p = make_params(**params)
+    r = Result()
+    _check(lib().trust_count_csr(pb, pa, len(beg) - 1, len(adj), C.byref(p), C.byref(r)))
+    return r
+
+
+def count_tiles(tiles, n: int, **params) -> Result:
+    arr = (Csr * (n * n))()
+    keep = []
+    for k, (b, a) in enumerate(tiles):
+        c, kk = _as_csr(b, a)
+        arr[k] = c
+        keep.append(kk)
+    p = make_params(**params)
+    r = Result()
+    _check(lib().trust_count_tiles(arr, n, C.byref(p), C.byref(r)))
+    return r
+
+
+def triangle_count(rank: int, folder: str, total_process: int = 1, threads: int = 0,
+                   blocks: int = 0, chunk_size: int = 0) -> Arguments:
+    a = Arguments()
+    _check(lib().trust_triangle_count(rank, folder.encode(), total_process, threads, blocks,
+                                      chunk_size, C.byref(a)))
+    return a
+
+
+def triangle_count_partition(rank: int, folder: str, total_process: int, threads: int,
+                             blocks: int, chunk_size: int, partition_num: int) -> Arguments:
+    a = Arguments()
+    _check(lib().trust_triangle_count_partition(rank, folder.encode(), total_process, threads,
+                                                blocks, chunk_size, partition_num, C.byref(a)))
+    return a
+
+
+class Context:
+    """A device context holding one resident graph (trust_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().trust_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.V = self.E = 0
+
+    def close(self):
+        if self._h:
+            lib().trust_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, beg, adj, flags: int = 0) -> float:
+        beg, pb = _i64(beg)
+        adj, pa = _i32(adj)
+        t = C.c_double(0)
+        _check(lib().trust_ctx_load_csr(self._h, pb, pa, len(beg) - 1, len(adj), flags,
+                                        C.byref(t)))
+        self.V, self.E = len(beg) - 1, len(adj)
+        return t.value
+
+    def count(self, **params) -> Result:
+        p = make_params(**params)
+        r = Result()
+        _check(lib().trust_ctx_count(self._h, C.byref(p), C.byref(r)))
+        return r
+
+    def profile_vertices(self, **params):
+        """Instrumented count: (cycles[V], probes[V], Result)."""
+        p = make_params(**params)
+        r = Result()
+        cyc = np.zeros(self.V, np.uint64)
+        prb = np.zeros(self.V, np.uint64)
+        _check(lib().trust_ctx_profile_vertices(self._h, C.byref(p), cyc.ctypes.data_as(_PU64),
+                                                prb.ctypes.data_as(_PU64), C.byref(r)))
+        return cyc, prb, r

@@ -1,0 +1,105 @@
+// Host-engine internals shared by engine.hip, partition_driver.hip,
+// learned.hip and batch.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+#include "tc_kernels.h"
+
+#define HIPCHK(expr)                                      \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess)                             \
+            return ::trust::hip_fail(_e, #expr);          \
+    } while (0)
+
+namespace trust {
+
+int hip_fail(hipError_t e, const char *what);
+
+// Grow-only device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+};
+
+// A CSR resident in HBM.  Offsets are int32 when E < 2^31 (halves the
+// offset traffic of the probe walk) unless int64 is requested.
+struct DevCsr {
+    DevBuf beg, adj;
+    int64_t V = 0, E = 0;
+    int64_t max_deg = 0;
+    bool off64 = false;
+};
+
+struct CsrViews {
+    const void *begH;
+    const int32_t *adjH;
+    const void *begW;
+    const int32_t *adjW;
+    const void *begP;
+    const int32_t *adjP;
+};
+
+struct LaunchShape {
+    int threads = 0, blocks = 0;
+    int wave_slots = 0, wave_lg = 0;
+    int table_words = 0, hub_lg = 0;
+    size_t lds_bytes = 0;
+    int chunk_group = 1, chunk_wave = 1;
+    long long spill_slots = 0;
+    int spill_groups = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int num_cus = 0;
+    size_t lds_max = 0;
+    DevCsr g;
+    double h2d_seconds = 0;
+    double learned_plan_seconds = 0;
+    DevBuf staging, list, block_counts, ctl, spill, instr_cycles, instr_probes;
+    // learned scheduler state
+    DevBuf feat, tiers, cost, mlp_params;
+    bool mlp_ready = false;
+    ~Ctx();
+};
+
+int ctx_init(Ctx *ctx, int device);
+int upload_csr(Ctx *ctx, DevCsr &g, const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,
+               bool force64, hipStream_t s);
+int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,
+                 uint32_t flags);
+int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_t V, bool off64,
+                   LaunchShape &L);
+int check_params(const trust_params &p);
+double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64);
+
+template <typename OffT>
+int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
+                  const trust_params &p, const LaunchShape &L, TcCtl *ctl_dev, bool instrument,
+                  unsigned long long *instr_cycles, unsigned long long *instr_probes,
+                  const unsigned char *tier_override, hipStream_t s, hipEvent_t ev_plan,
+                  hipEvent_t ev_count);
+
+int count_resident(Ctx *ctx, const trust_params &p, bool instrument,
+                   unsigned long long *cycles_out, unsigned long long *probes_out,
+                   trust_result *r);
+
+// Learned scheduler (learned.hip): fills a per-vertex tier array on the
+// device for the resident graph and returns it through *tiers.
+int learned_plan(Ctx *ctx, const trust_params &p, const LaunchShape &L,
+                 const unsigned char **tiers);
+
+}  // namespace trust

@@ -1,0 +1,513 @@
+// Host engine of libtrust_amd.so: device contexts, launch-shape choice,
+// planning + counting launches, and the C-ABI counting entry points.
+//
+// Replaces the host half of the reference (Triangle_count,
+// Without-graph-partition/TC.cu:602-731 and With-graph-partition/TC.cu:976-1157):
+// the reference loads, allocates, copies, launches and frees inside every
+// call and exits on any CUDA error; here a context keeps the graph and the
+// scratch buffers resident, every failure is an error code, and times come
+// from hipEvents on the launch stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "engine.h"
+
+namespace trust {
+
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(TRUST_ERR_OOM, "%s: %s", what, hipGetErrorString(e));
+    return fail(TRUST_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int DevBuf::ensure(size_t n)
+{
+    if (n <= bytes)
+        return TRUST_OK;
+    release();
+    if (n == 0)
+        return TRUST_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    bytes = n;
+    return TRUST_OK;
+}
+
+void DevBuf::release()
+{
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int upload_csr(Ctx *ctx, DevCsr &g, const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,
+               bool force64, hipStream_t s)
+{
+    g.V = V;
+    g.E = E;
+    g.off64 = force64 || E >= (int64_t)INT32_MAX;
+    g.max_deg = host_max_degree(beg, V);
+    int rc = g.adj.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1));
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(g.adj.p, adj, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice, s));
+    if (g.off64) {
+        rc = g.beg.ensure(sizeof(int64_t) * (size_t)(V + 1));
+        if (rc)
+            return rc;
+        HIPCHK(hipMemcpyAsync(g.beg.p, beg, sizeof(int64_t) * (size_t)(V + 1),
+                              hipMemcpyHostToDevice, s));
+    } else {
+        rc = g.beg.ensure(sizeof(int32_t) * (size_t)(V + 1));
+        if (rc)
+            return rc;
+        rc = ctx->staging.ensure(sizeof(int64_t) * (size_t)(V + 1));
+        if (rc)
+            return rc;
+        HIPCHK(hipMemcpyAsync(ctx->staging.p, beg, sizeof(int64_t) * (size_t)(V + 1),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(launch_offsets_to_i32((const int64_t *)ctx->staging.p, (int32_t *)g.beg.p, V + 1, s));
+    }
+    return TRUST_OK;
+}
+
+static bool is_pow2(long long x) { return x > 0 && (x & (x - 1)) == 0; }
+static int floor_lg(long long x)
+{
+    int lg = 0;
+    while ((2ll << lg) <= x)
+        lg++;
+    return lg;
+}
+
+int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_t V,
+                   bool off64, LaunchShape &L)
+{
+    L.threads = p.threads > 0 ? p.threads : 512;
+    if (L.threads % kWave != 0 || L.threads > 1024)
+        return fail(TRUST_ERR_ARG, "threads=%d: must be a multiple of 64 and at most 1024",
+                    L.threads);
+    const int nwaves = L.threads / kWave;
+    L.wave_slots = p.wave_slots > 0 ? p.wave_slots : 512;
+    if (!is_pow2(L.wave_slots) || L.wave_slots < 64)
+        return fail(TRUST_ERR_ARG, "wave_slots=%d: must be a power of two >= 64", L.wave_slots);
+    L.wave_lg = floor_lg(L.wave_slots);
+    const int extra = lds_extra_words(L.threads);
+    const long long lds_max = (long long)ctx->lds_max;
+    long long budget = p.lds_budget > 0 ? std::min<long long>(p.lds_budget, lds_max)
+                                        : std::min<long long>(64 * 1024 + extra * 4ll, lds_max);
+    long long budget_words = budget / 4 - extra;
+    const long long wave_words = (long long)nwaves * L.wave_slots;
+    if (budget_words < wave_words)
+        return fail(TRUST_ERR_ARG,
+                    "LDS budget %lld bytes cannot hold %d wave tables of %d slots", budget,
+                    nwaves, L.wave_slots);
+    long long need = 1ll << host_table_lg(std::max<int64_t>(max_deg_hash, 1));
+    long long hub_words = std::min<long long>(need, 1ll << floor_lg(budget_words));
+    L.table_words = (int)std::max(wave_words, hub_words);
+    L.hub_lg = floor_lg(L.table_words);
+    L.lds_bytes = (size_t)(L.table_words + extra) * 4;
+    int per_cu = 0;
+    hipError_t e = off64 ? count_occupancy<int64_t>(L.threads, L.lds_bytes, false, &per_cu)
+                         : count_occupancy<int32_t>(L.threads, L.lds_bytes, false, &per_cu);
+    if (e != hipSuccess)
+        return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (per_cu < 1)
+        return fail(TRUST_ERR_ARG, "launch of %d threads with %zu LDS bytes cannot run",
+                    L.threads, L.lds_bytes);
+    L.blocks = p.blocks > 0 ? p.blocks : per_cu * ctx->num_cus;
+    if (p.chunk > 0) {
+        L.chunk_group = L.chunk_wave = p.chunk;
+    } else {
+        L.chunk_group = 1;
+        long long waves = (long long)L.blocks * nwaves;
+        L.chunk_wave = (int)std::max<long long>(1, std::min<long long>(16, V / (waves * 64 + 1)));
+    }
+    // HBM tables for vertices whose table exceeds the LDS hub table.
+    const int max_lg = host_table_lg(std::max<int64_t>(max_deg_hash, 1));
+    const bool any_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) || max_lg > L.hub_lg;
+    if (any_spill) {
+        L.spill_slots = 1ll << max_lg;
+        const long long spill_budget = 2ll << 30;  // bytes of HBM for spill tables
+        long long groups = spill_budget / (L.spill_slots * 4);
+        L.spill_groups = (int)std::max<long long>(1, std::min<long long>(groups, L.blocks));
+    } else {
+        L.spill_slots = 0;
+        L.spill_groups = 0;
+    }
+    return TRUST_OK;
+}
+
+template <typename OffT>
+int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
+                  const trust_params &p, const LaunchShape &L, TcCtl *ctl_dev, bool instrument,
+                  unsigned long long *instr_cycles, unsigned long long *instr_probes,
+                  const unsigned char *tier_override, hipStream_t s, hipEvent_t ev_plan,
+                  hipEvent_t ev_count)
+{
+    int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
+    if (rc)
+        return rc;
+    const int per_block = kPlanPerBlock;
+    const int nblocks = (int)((V + per_block - 1) / per_block);
+    rc = ctx->block_counts.ensure(sizeof(long long) * (size_t)kNumTiers * std::max(nblocks, 1));
+    if (rc)
+        return rc;
+    if (L.spill_groups > 0) {
+        rc = ctx->spill.ensure(sizeof(int32_t) * (size_t)L.spill_slots * L.spill_groups);
+        if (rc)
+            return rc;
+    }
+    HIPCHK(launch_fill_u32((uint32_t *)ctl_dev, 0u, sizeof(TcCtl) / 4, s));
+
+    PlanArgs<OffT> pa;
+    memset(&pa, 0, sizeof(pa));
+    pa.begH = (const OffT *)views.begH;
+    pa.begW = (const OffT *)views.begW;
+    pa.V = V;
+    pa.same_csr = same_csr ? 1 : 0;
+    pa.hub_threshold = p.hub_threshold;
+    pa.wave_lg = L.wave_lg;
+    pa.hub_lg = L.hub_lg;
+    pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
+    pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
+    pa.tier_override = tier_override;
+    pa.list = (int32_t *)ctx->list.p;
+    pa.block_counts = (long long *)ctx->block_counts.p;
+    pa.per_block = per_block;
+    pa.ctl = ctl_dev;
+    HIPCHK(launch_plan<OffT>(pa, nblocks, s));
+    if (ev_plan)
+        HIPCHK(hipEventRecord(ev_plan, s));
+
+    TcArgs<OffT> a;
+    memset(&a, 0, sizeof(a));
+    a.begH = (const OffT *)views.begH;
+    a.adjH = views.adjH;
+    a.begW = (const OffT *)views.begW;
+    a.adjW = views.adjW;
+    a.begP = (const OffT *)views.begP;
+    a.adjP = views.adjP;
+    a.list = (const int32_t *)ctx->list.p;
+    a.spill_mem = (int32_t *)ctx->spill.p;
+    a.spill_slots = L.spill_slots;
+    a.spill_groups = L.spill_groups;
+    a.table_words = L.table_words;
+    a.hub_lg = L.hub_lg;
+    a.wave_slots = L.wave_slots;
+    a.chunk_group = L.chunk_group;
+    a.chunk_wave = L.chunk_wave;
+    a.rank = p.rank;
+    a.nranks = p.total_process;
+    a.ctl = ctl_dev;
+    a.instr_cycles = instr_cycles;
+    a.instr_probes = instr_probes;
+    HIPCHK(launch_count<OffT>(a, L.blocks, L.threads, L.lds_bytes, instrument, s));
+    if (ev_count)
+        HIPCHK(hipEventRecord(ev_count, s));
+    return TRUST_OK;
+}
+
+template int enqueue_count<int32_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
+                                    const LaunchShape &, TcCtl *, bool, unsigned long long *,
+                                    unsigned long long *, const unsigned char *, hipStream_t,
+                                    hipEvent_t, hipEvent_t);
+template int enqueue_count<int64_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
+                                    const LaunchShape &, TcCtl *, bool, unsigned long long *,
+                                    unsigned long long *, const unsigned char *, hipStream_t,
+                                    hipEvent_t, hipEvent_t);
+
+int check_params(const trust_params &p)
+{
+    if (p.total_process < 1 || p.rank < 0 || p.rank >= p.total_process)
+        return fail(TRUST_ERR_ARG, "rank %d of total_process %d is invalid", p.rank,
+                    p.total_process);
+    if (p.hub_threshold < 0)
+        return fail(TRUST_ERR_ARG, "hub_threshold %d < 0", p.hub_threshold);
+    if (p.chunk < 0 || p.blocks < 0 || p.threads < 0)
+        return fail(TRUST_ERR_ARG, "negative launch parameter");
+    if (p.schedule != TRUST_SCHED_DEGREE && p.schedule != TRUST_SCHED_LEARNED)
+        return fail(TRUST_ERR_ARG, "unknown schedule %d", p.schedule);
+    return TRUST_OK;
+}
+
+double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
+{
+    // offsets once + every N+(u) once + two offsets per walked v + every probe
+    const double ob = off64 ? 8.0 : 4.0;
+    return ob * (double)(V + 1) + 4.0 * (double)E + 2.0 * ob * (double)E + 4.0 * (double)W;
+}
+
+// Count over the context's resident graph (or a caller-provided view set).
+int count_resident(Ctx *ctx, const trust_params &p, bool instrument,
+                   unsigned long long *cycles_out, unsigned long long *probes_out,
+                   trust_result *r)
+{
+    int rc = check_params(p);
+    if (rc)
+        return rc;
+    DevCsr &g = ctx->g;
+    if (!g.adj.p && g.E > 0)
+        return fail(TRUST_ERR_ARG, "context holds no graph");
+    if (g.max_deg > (int64_t)(INT32_MAX / 64))
+        return fail(TRUST_ERR_OVERFLOW, "max out-degree %lld too large for one probe group",
+                    (long long)g.max_deg);
+    LaunchShape L;
+    rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, L);
+    if (rc)
+        return rc;
+    rc = ctx->ctl.ensure(sizeof(TcCtl));
+    if (rc)
+        return rc;
+    if (instrument) {
+        rc = ctx->instr_cycles.ensure(sizeof(unsigned long long) * (size_t)std::max<int64_t>(g.V, 1));
+        if (!rc)
+            rc = ctx->instr_probes.ensure(sizeof(unsigned long long) *
+                                          (size_t)std::max<int64_t>(g.V, 1));
+        if (rc)
+            return rc;
+        HIPCHK(hipMemsetAsync(ctx->instr_cycles.p, 0, sizeof(unsigned long long) * g.V, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->instr_probes.p, 0, sizeof(unsigned long long) * g.V, ctx->stream));
+    }
+    const unsigned char *override_tiers = nullptr;
+    if (p.schedule == TRUST_SCHED_LEARNED) {
+        rc = learned_plan(ctx, p, L, &override_tiers);
+        if (rc)
+            return rc;
+    }
+    CsrViews v{g.beg.p, (const int32_t *)g.adj.p, g.beg.p, (const int32_t *)g.adj.p, g.beg.p,
+               (const int32_t *)g.adj.p};
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (g.off64)
+        rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, instrument,
+                                    (unsigned long long *)ctx->instr_cycles.p,
+                                    (unsigned long long *)ctx->instr_probes.p, override_tiers,
+                                    ctx->stream, ctx->ev[1], ctx->ev[2]);
+    else
+        rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, instrument,
+                                    (unsigned long long *)ctx->instr_cycles.p,
+                                    (unsigned long long *)ctx->instr_probes.p, override_tiers,
+                                    ctx->stream, ctx->ev[1], ctx->ev[2]);
+    if (rc)
+        return rc;
+    TcCtl h;
+    HIPCHK(hipMemcpyAsync(&h, ctx->ctl.p, sizeof(TcCtl), hipMemcpyDeviceToHost, ctx->stream));
+    if (instrument) {
+        if (cycles_out)
+            HIPCHK(hipMemcpyAsync(cycles_out, ctx->instr_cycles.p,
+                                  sizeof(unsigned long long) * g.V, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        if (probes_out)
+            HIPCHK(hipMemcpyAsync(probes_out, ctx->instr_probes.p,
+                                  sizeof(unsigned long long) * g.V, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float t_plan = 0, t_count = 0;
+    HIPCHK(hipEventElapsedTime(&t_plan, ctx->ev[0], ctx->ev[1]));
+    HIPCHK(hipEventElapsedTime(&t_count, ctx->ev[1], ctx->ev[2]));
+    if (r) {
+        memset(r, 0, sizeof(*r));
+        r->vertices = g.V;
+        r->edges = g.E;
+        r->triangles = h.count;
+        r->probes = h.probes;
+        r->plan_seconds = t_plan * 1e-3 + ctx->learned_plan_seconds;
+        r->count_seconds = t_count * 1e-3;
+        r->seconds = r->plan_seconds + r->count_seconds;
+        r->h2d_seconds = ctx->h2d_seconds;
+        r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
+        r->alg_bytes = alg_bytes(g.V, g.E, h.probes, g.off64);
+        r->n_spill = h.list_begin[1] - h.list_begin[0];
+        r->n_hub = h.list_begin[2] - h.list_begin[1];
+        r->n_wave = h.list_begin[3] - h.list_begin[2];
+        r->threads = L.threads;
+        r->blocks = L.blocks;
+        r->hub_slots = 1 << L.hub_lg;
+        r->wave_slots = L.wave_slots;
+    }
+    ctx->learned_plan_seconds = 0;
+    return TRUST_OK;
+}
+
+int ctx_init(Ctx *ctx, int device)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(TRUST_ERR_NODEV, "no HIP device available (%s)",
+                    e == hipSuccess ? "count 0" : hipGetErrorString(e));
+    if (device < 0 || device >= n)
+        return fail(TRUST_ERR_ARG, "device %d out of range [0,%d)", device, n);
+    ctx->device = device;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->lds_max = prop.sharedMemPerBlock;
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    for (auto &ev : ctx->ev)
+        HIPCHK(hipEventCreate(&ev));
+    ctx->ready = true;
+    return TRUST_OK;
+}
+
+Ctx::~Ctx()
+{
+    if (!ready)
+        return;
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(copy_stream);
+    g.beg.release();
+    g.adj.release();
+    for (auto &ev : this->ev)
+        (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(stream);
+    (void)hipStreamDestroy(copy_stream);
+}
+
+int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,
+                 uint32_t flags)
+{
+    if (!(flags & TRUST_FLAG_NO_VALIDATE)) {
+        int rc = trust_csr_validate(beg, adj, V, E);
+        if (rc)
+            return rc;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = upload_csr(ctx, ctx->g, beg, adj, V, E, (flags & TRUST_FLAG_OFFSETS64) != 0,
+                        ctx->stream);
+    if (rc)
+        return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->h2d_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return TRUST_OK;
+}
+
+}  // namespace trust
+
+using namespace trust;
+
+extern "C" int trust_device_count(void)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice)
+        return 0;
+    if (e != hipSuccess)
+        return hip_fail(e, "hipGetDeviceCount");
+    return n;
+}
+
+extern "C" int trust_ctx_create(int device, trust_ctx **out)
+{
+    if (!out)
+        return fail(TRUST_ERR_ARG, "null output");
+    *out = nullptr;
+    std::unique_ptr<Ctx> c(new (std::nothrow) Ctx());
+    if (!c)
+        return fail(TRUST_ERR_OOM, "cannot allocate context");
+    int rc = ctx_init(c.get(), device);
+    if (rc)
+        return rc;
+    *out = reinterpret_cast<trust_ctx *>(c.release());
+    return TRUST_OK;
+}
+
+extern "C" void trust_ctx_destroy(trust_ctx *ctx) { delete reinterpret_cast<Ctx *>(ctx); }
+
+extern "C" int trust_ctx_load_csr(trust_ctx *c, const int64_t *beg, const int32_t *adj,
+                                  int64_t V, int64_t E, uint32_t flags, double *h2d_seconds)
+{
+    if (!c)
+        return fail(TRUST_ERR_ARG, "null context");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    int rc = load_ctx_csr(ctx, beg, adj, V, E, flags);
+    if (!rc && h2d_seconds)
+        *h2d_seconds = ctx->h2d_seconds;
+    return rc;
+}
+
+extern "C" int trust_ctx_count(trust_ctx *c, const trust_params *p, trust_result *r)
+{
+    if (!c || !p)
+        return fail(TRUST_ERR_ARG, "null argument");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    HIPCHK(hipSetDevice(ctx->device));
+    return count_resident(ctx, *p, false, nullptr, nullptr, r);
+}
+
+extern "C" int trust_ctx_profile_vertices(trust_ctx *c, const trust_params *p,
+                                          uint64_t *cycles, uint64_t *probes, trust_result *r)
+{
+    if (!c || !p)
+        return fail(TRUST_ERR_ARG, "null argument");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    HIPCHK(hipSetDevice(ctx->device));
+    return count_resident(ctx, *p, true, (unsigned long long *)cycles,
+                          (unsigned long long *)probes, r);
+}
+
+extern "C" int trust_count_csr(const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,
+                               const trust_params *p, trust_result *r)
+{
+    if (!p)
+        return fail(TRUST_ERR_ARG, "null params");
+    Ctx ctx;
+    int rc = ctx_init(&ctx, p->device);
+    if (rc)
+        return rc;
+    rc = load_ctx_csr(&ctx, beg, adj, V, E, p->flags);
+    if (rc)
+        return rc;
+    return count_resident(&ctx, *p, false, nullptr, nullptr, r);
+}
+
+extern "C" int trust_triangle_count(int rank, const char *folder, int total_process, int threads,
+                                    int blocks, int chunk_size, trust_arguments *out)
+{
+    if (!folder || !out)
+        return fail(TRUST_ERR_ARG, "null argument");
+    HostCsr g;
+    int rc = load_folder(folder, g);
+    if (rc)
+        return rc;
+    trust_params p;
+    trust_params_init(&p);
+    p.rank = rank;
+    p.total_process = total_process;
+    p.threads = threads > 0 ? threads : 0;
+    p.blocks = blocks > 0 ? blocks : 0;
+    p.chunk = chunk_size > 0 ? chunk_size : 0;
+    Ctx ctx;
+    rc = ctx_init(&ctx, 0);
+    if (rc)
+        return rc;
+    rc = load_ctx_csr(&ctx, g.beg.data(), g.adj.data(), g.V, g.E, TRUST_FLAG_NO_VALIDATE);
+    if (rc)
+        return rc;
+    trust_result r;
+    rc = count_resident(&ctx, p, false, nullptr, nullptr, &r);
+    if (rc)
+        return rc;
+    out->edge_count = g.E;
+    out->count = r.triangles;
+    out->time = r.seconds;
+    out->degree = g.V > 0 ? g.E / g.V : 0;
+    out->vertices = g.V;
+    return TRUST_OK;
+}

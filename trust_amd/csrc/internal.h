@@ -1,0 +1,49 @@
+// Internal declarations shared by the host-side translation units of
+// libtrust_amd.so.  Not part of the public ABI (include/trust.h is).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "trust.h"
+
+namespace trust {
+
+// Record a message for trust_last_error() and return `code`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// Owned host CSR.
+struct HostCsr {
+    int64_t V = 0;
+    int64_t E = 0;
+    std::vector<int64_t> beg;
+    std::vector<int32_t> adj;
+};
+
+// Hand an owned CSR to a caller-visible trust_csr (malloc'd buffers).
+int export_csr(HostCsr &&src, trust_csr *out);
+
+// File helpers.
+int64_t file_size(const std::string &path);
+int read_file(const std::string &path, void *dst, size_t bytes);
+int write_file(const std::string &path, const void *src, size_t bytes);
+int load_folder(const std::string &folder, HostCsr &out);
+
+// Largest beg[u+1]-beg[u] (OpenMP, host).
+int64_t host_max_degree(const int64_t *beg, int64_t V);
+
+// Parallel LSD radix sort of 64-bit keys (stable); tmp must hold n keys.
+void radix_sort_u64(uint64_t *keys, uint64_t *tmp, size_t n, int key_bits);
+
+// Input contract (reference preprocessing), see preprocess.cpp.
+int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr &out,
+                     int64_t n_fixed = -1);
+
+// 2D hash partition tiles, see partition.cpp.
+int partition_tiles(const int64_t *beg, const int32_t *adj, int64_t V, int n,
+                    std::vector<HostCsr> &tiles);
+
+}  // namespace trust

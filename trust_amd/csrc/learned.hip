@@ -1,0 +1,16 @@
+// Learned cost scheduler (placeholder until the MLP kernels land).
+#include "engine.h"
+
+namespace trust {
+
+int learned_plan(Ctx *ctx, const trust_params &p, const LaunchShape &L,
+                 const unsigned char **tiers)
+{
+    (void)ctx;
+    (void)p;
+    (void)L;
+    *tiers = nullptr;
+    return fail(TRUST_ERR_ARG, "learned schedule not available in this build");
+}
+
+}  // namespace trust

@@ -1,0 +1,384 @@
+// 2D hash-partition path on one GPU.
+//
+// A job (i,j,k) hashes tile (i,j), walks tile (i,k) and probes tile (k,j)
+// (With-graph-partition/TC.cu:985-994); summing the n^3 jobs gives the
+// whole count because a triangle u->v->w (u->w) has exactly one (u%n, w%n,
+// v%n).  The reference runs one job per MPI rank and copies its three
+// tiles synchronously each time (TC.cu:1034-1077).  Here all jobs stream
+// through one device: tiles live in a bounded device cache, missing tiles
+// are copied on a second stream while the previous job counts, and each
+// job's uint64 lands in its own control block, summed once at the end.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace trust {
+
+// Rows of tile (i, *) hash/walk vertices u with u % n == i; entries of a
+// walk tile index rows of the probe tile.  HT and probe entries are only
+// compared for equality, so they only need to be valid (non-negative) keys.
+static int check_job_tiles(const trust_csr &ht, const trust_csr &walk, const trust_csr &probe)
+{
+    const trust_csr *t[3] = {&ht, &walk, &probe};
+    for (int q = 0; q < 3; q++) {
+        const trust_csr &c = *t[q];
+        if (c.vertices < 0 || !c.beg || (c.edges > 0 && !c.adj))
+            return fail(TRUST_ERR_ARG, "tile %d is empty or null", q);
+        if (c.beg[0] != 0 || c.beg[c.vertices] != c.edges)
+            return fail(TRUST_ERR_FORMAT, "tile %d: offsets do not span its %lld entries", q,
+                        (long long)c.edges);
+        for (int64_t u = 0; u < c.vertices; u++)
+            if (c.beg[u + 1] < c.beg[u])
+                return fail(TRUST_ERR_FORMAT, "tile %d: offsets decrease at row %lld", q,
+                            (long long)u);
+    }
+    if (ht.vertices != walk.vertices)
+        return fail(TRUST_ERR_FORMAT, "hash tile has %lld rows but walk tile has %lld",
+                    (long long)ht.vertices, (long long)walk.vertices);
+    int64_t bad = -1;
+    for (int64_t e = 0; e < walk.edges && bad < 0; e++)
+        if (walk.adj[e] < 0 || walk.adj[e] >= probe.vertices)
+            bad = e;
+    if (bad >= 0)
+        return fail(TRUST_ERR_FORMAT, "walk tile entry %lld = %d is not a row of the probe tile",
+                    (long long)bad, walk.adj[bad]);
+    for (int q = 0; q < 3; q += 2)
+        for (int64_t e = 0; e < t[q]->edges; e++)
+            if (t[q]->adj[e] < 0)
+                return fail(TRUST_ERR_FORMAT, "tile %d holds a negative id", q);
+    return TRUST_OK;
+}
+
+namespace {
+
+struct TileSlot {
+    DevCsr g;
+    int tile = -1;                 // i*n+j held, -1 when empty
+    hipEvent_t ready = nullptr;    // copy finished (copy stream)
+    hipEvent_t released = nullptr; // last job reading it finished (compute stream)
+    long long last_job = -1;
+};
+
+}  // namespace
+
+// Streams all jobs of an n x n partition; tiles[i*n+j] is tile (i,j).
+// job_counts (nullable, n^3 entries) receives each job's count.
+static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_params &p,
+                        int cache_slots, uint64_t *job_counts, trust_result *r)
+{
+    const long long njobs = (long long)n * n * n;
+    bool off64 = (p.flags & TRUST_FLAG_OFFSETS64) != 0;
+    int64_t maxd = 0, Vsum = 0, Esum = 0;
+    for (int t = 0; t < n * n; t++) {
+        off64 = off64 || tiles[t].edges >= (int64_t)INT32_MAX;
+        for (int64_t u = 0; u < tiles[t].vertices; u++)
+            maxd = std::max(maxd, tiles[t].beg[u + 1] - tiles[t].beg[u]);
+        Esum += tiles[t].edges;
+    }
+    for (int i = 0; i < n; i++)
+        Vsum += tiles[(size_t)i * n].vertices;
+    if (maxd > (int64_t)(INT32_MAX / 64))
+        return fail(TRUST_ERR_OVERFLOW, "tile degree %lld too large", (long long)maxd);
+    if (!(p.flags & TRUST_FLAG_NO_VALIDATE))
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++)
+                for (int k = 0; k < n; k++) {
+                    int rc = check_job_tiles(tiles[i * n + j], tiles[i * n + k], tiles[k * n + j]);
+                    if (rc)
+                        return rc;
+                }
+    LaunchShape L;
+    int rc = resolve_launch(ctx, p, maxd, Vsum, off64, L);
+    if (rc)
+        return rc;
+    cache_slots = std::max(3, std::min(cache_slots, n * n));
+    std::vector<TileSlot> slots((size_t)cache_slots);
+    for (auto &s : slots) {
+        HIPCHK(hipEventCreateWithFlags(&s.ready, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&s.released, hipEventDisableTiming));
+    }
+    // Size every device buffer for the largest tile up front: a buffer is
+    // never reallocated while a queued copy or count may still use it.
+    int64_t maxV = 0, maxE = 0;
+    for (int t = 0; t < n * n; t++) {
+        maxV = std::max(maxV, tiles[t].vertices);
+        maxE = std::max(maxE, tiles[t].edges);
+    }
+    rc = ctx->staging.ensure(sizeof(int64_t) * (size_t)(maxV + 1));
+    if (rc)
+        return rc;
+    for (auto &s : slots) {
+        rc = s.g.beg.ensure((off64 ? 8 : 4) * (size_t)(maxV + 1));
+        if (!rc)
+            rc = s.g.adj.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(maxE, 1));
+        if (rc)
+            return rc;
+    }
+    rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(maxV, 1));
+    if (!rc)
+        rc = ctx->block_counts.ensure(sizeof(long long) * kNumTiers *
+                                      (size_t)std::max<int64_t>((maxV + kPlanPerBlock - 1) / kPlanPerBlock, 1));
+    if (rc)
+        return rc;
+    struct SlotGuard {
+        std::vector<TileSlot> &s;
+        ~SlotGuard()
+        {
+            for (auto &x : s) {
+                if (x.ready)
+                    (void)hipEventDestroy(x.ready);
+                if (x.released)
+                    (void)hipEventDestroy(x.released);
+            }
+        }
+    } guard{slots};
+    DevBuf ctls;
+    rc = ctls.ensure(sizeof(TcCtl) * (size_t)njobs);
+    if (rc)
+        return rc;
+
+    auto job_tiles = [n](long long job, int out[3]) {
+        int i = (int)(job / ((long long)n * n)), j = (int)(job / n % n), k = (int)(job % n);
+        out[0] = i * n + j;  // hash
+        out[1] = i * n + k;  // walk
+        out[2] = k * n + j;  // probe
+    };
+    // Make tile t resident for job `job`; never evicts tiles of jobs >= pin_from.
+    auto acquire = [&](int t, long long job, const int keep[6], int &slot_out) -> int {
+        for (int s = 0; s < cache_slots; s++)
+            if (slots[s].tile == t) {
+                slot_out = s;
+                return TRUST_OK;
+            }
+        int victim = -1;
+        for (int s = 0; s < cache_slots; s++) {
+            bool pinned = false;
+            for (int q = 0; q < 6; q++)
+                pinned = pinned || (keep[q] >= 0 && slots[s].tile == keep[q]);
+            if (pinned)
+                continue;
+            if (victim < 0 || slots[s].last_job < slots[victim].last_job)
+                victim = s;
+        }
+        if (victim < 0)
+            return fail(TRUST_ERR_ARG, "tile cache of %d slots too small", cache_slots);
+        TileSlot &v = slots[victim];
+        if (v.last_job >= 0)
+            HIPCHK(hipStreamWaitEvent(ctx->copy_stream, v.released, 0));
+        const trust_csr &src = tiles[t];
+        int rc2 = upload_csr(ctx, v.g, src.beg, src.adj, src.vertices, src.edges, off64,
+                             ctx->copy_stream);
+        if (rc2)
+            return rc2;
+        HIPCHK(hipEventRecord(v.ready, ctx->copy_stream));
+        v.tile = t;
+        v.last_job = job;
+        slot_out = victim;
+        (void)job;
+        return TRUST_OK;
+    };
+
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    for (long long job = 0; job < njobs; job++) {
+        int cur[3], nxt[3] = {-1, -1, -1};
+        job_tiles(job, cur);
+        if (job + 1 < njobs)
+            job_tiles(job + 1, nxt);
+        const int keep[6] = {cur[0], cur[1], cur[2], -1, -1, -1};
+        int sl[3];
+        for (int q = 0; q < 3; q++) {
+            rc = acquire(cur[q], job, keep, sl[q]);
+            if (rc)
+                return rc;
+        }
+        for (int q = 0; q < 3; q++) {
+            slots[sl[q]].last_job = job;
+            HIPCHK(hipStreamWaitEvent(ctx->stream, slots[sl[q]].ready, 0));
+        }
+        CsrViews v{slots[sl[0]].g.beg.p, (const int32_t *)slots[sl[0]].g.adj.p,
+                   slots[sl[1]].g.beg.p, (const int32_t *)slots[sl[1]].g.adj.p,
+                   slots[sl[2]].g.beg.p, (const int32_t *)slots[sl[2]].g.adj.p};
+        const int64_t rows = tiles[cur[0]].vertices;
+        TcCtl *ctl = (TcCtl *)ctls.p + job;
+        rc = off64 ? enqueue_count<int64_t>(ctx, v, rows, false, p, L, ctl, false, nullptr,
+                                            nullptr, nullptr, ctx->stream, nullptr, nullptr)
+                   : enqueue_count<int32_t>(ctx, v, rows, false, p, L, ctl, false, nullptr,
+                                            nullptr, nullptr, ctx->stream, nullptr, nullptr);
+        if (rc)
+            return rc;
+        for (int q = 0; q < 3; q++)
+            HIPCHK(hipEventRecord(slots[sl[q]].released, ctx->stream));
+        // Prefetch the next job's tiles while this one counts.
+        if (job + 1 < njobs) {
+            const int keep2[6] = {cur[0], cur[1], cur[2], nxt[0], nxt[1], nxt[2]};
+            for (int q = 0; q < 3; q++) {
+                int dummy;
+                bool fits = cache_slots >= 6;
+                if (!fits)
+                    break;  // too few slots to prefetch without evicting the running job
+                rc = acquire(nxt[q], job + 1, keep2, dummy);
+                if (rc)
+                    return rc;
+            }
+        }
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    std::vector<TcCtl> h((size_t)njobs);
+    HIPCHK(hipMemcpyAsync(h.data(), ctls.p, sizeof(TcCtl) * (size_t)njobs, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    uint64_t total = 0, probes = 0;
+    for (long long j = 0; j < njobs; j++) {
+        total += h[j].count;
+        probes += h[j].probes;
+        if (job_counts)
+            job_counts[j] = h[j].count;
+    }
+    if (r) {
+        memset(r, 0, sizeof(*r));
+        r->vertices = Vsum;
+        r->edges = Esum;
+        r->triangles = total;
+        r->probes = probes;
+        r->seconds = ms * 1e-3;
+        r->count_seconds = r->seconds;
+        r->gteps = r->seconds > 0 ? (double)Esum / r->seconds / 1e9 : 0;
+        r->alg_bytes = alg_bytes(Vsum, Esum, probes, off64);
+        r->threads = L.threads;
+        r->blocks = L.blocks;
+        r->hub_slots = 1 << L.hub_lg;
+        r->wave_slots = L.wave_slots;
+    }
+    return TRUST_OK;
+}
+
+}  // namespace trust
+
+using namespace trust;
+
+extern "C" int trust_count_tiles(const trust_csr *tiles, int n, const trust_params *p,
+                                 trust_result *r)
+{
+    if (!tiles || !p || n < 1)
+        return fail(TRUST_ERR_ARG, "bad arguments");
+    int rc = check_params(*p);
+    if (rc)
+        return rc;
+    Ctx ctx;
+    rc = ctx_init(&ctx, p->device);
+    if (rc)
+        return rc;
+    // reserved[0]: device tile-cache slots (0 = 2n+2, enough to prefetch)
+    int slots = p->reserved[0] > 0 ? p->reserved[0] : 2 * n + 2;
+    return stream_tiles(&ctx, tiles, n, *p, slots, nullptr, r);
+}
+
+extern "C" int trust_triangle_count_partition(int rank, const char *folder, int total_process,
+                                              int threads, int blocks, int chunk_size,
+                                              int partition_num, trust_arguments *out)
+{
+    if (!folder || !out || partition_num < 1)
+        return fail(TRUST_ERR_ARG, "bad arguments");
+    const int n = partition_num;
+    const long long n3 = (long long)n * n * n;
+    if (total_process < n3 || total_process % n3 != 0)
+        return fail(TRUST_ERR_ARG,
+                    "total_process %d must be a positive multiple of n^3 = %lld (one job per "
+                    "(i,j,k) tile triple)",
+                    total_process, n3);
+    if (rank < 0 || rank >= total_process)
+        return fail(TRUST_ERR_ARG, "rank %d out of range", rank);
+    const int i = rank % n, j = rank / n % n, k = rank / n / n % n;
+    auto tile_dir = [&](int a, int b) {
+        return std::string(folder) + "/partition" + std::to_string(a) + "_" + std::to_string(b);
+    };
+    HostCsr g[3];
+    const std::string dirs[3] = {tile_dir(i, j), tile_dir(i, k), tile_dir(k, j)};
+    for (int q = 0; q < 3; q++) {
+        const std::string fb = dirs[q] + "/begin.bin", fa = dirs[q] + "/adjacent.bin";
+        int64_t sb = file_size(fb), sa = file_size(fa);
+        if (sb < 8 || sb % 8 || sa < 0 || sa % 4)
+            return fail(TRUST_ERR_IO, "tile %s is missing or malformed", dirs[q].c_str());
+        g[q].V = sb / 8 - 1;
+        g[q].E = sa / 4;
+        g[q].beg.resize((size_t)g[q].V + 1);
+        g[q].adj.resize((size_t)g[q].E);
+        int rc = read_file(fb, g[q].beg.data(), (size_t)sb);
+        if (!rc)
+            rc = read_file(fa, g[q].adj.data(), (size_t)sa);
+        if (rc)
+            return rc;
+    }
+    trust_csr t[3];
+    for (int q = 0; q < 3; q++)
+        t[q] = trust_csr{g[q].V, g[q].E, g[q].beg.data(), g[q].adj.data()};
+    int rc = check_job_tiles(t[0], t[1], t[2]);
+    if (rc)
+        return rc;
+    trust_params p;
+    trust_params_init(&p);
+    p.rank = (int)(rank / n3);
+    p.total_process = (int)(total_process / n3);
+    p.threads = threads > 0 ? threads : 0;
+    p.blocks = blocks > 0 ? blocks : 0;
+    p.chunk = chunk_size > 0 ? chunk_size : 0;
+    Ctx ctx;
+    rc = ctx_init(&ctx, 0);
+    if (rc)
+        return rc;
+    bool off64 = false;
+    int64_t maxd = 0;
+    for (int q = 0; q < 3; q++)
+        off64 = off64 || t[q].edges >= (int64_t)INT32_MAX;
+    for (int64_t u = 0; u < t[0].vertices; u++)
+        maxd = std::max(maxd, t[0].beg[u + 1] - t[0].beg[u]);
+    DevCsr d[3];
+    for (int q = 0; q < 3; q++) {
+        rc = upload_csr(&ctx, d[q], t[q].beg, t[q].adj, t[q].vertices, t[q].edges, off64,
+                        ctx.stream);
+        if (rc)
+            return rc;
+        HIPCHK(hipStreamSynchronize(ctx.stream));
+    }
+    LaunchShape L;
+    rc = resolve_launch(&ctx, p, maxd, t[0].vertices, off64, L);
+    if (rc)
+        return rc;
+    rc = ctx.ctl.ensure(sizeof(TcCtl));
+    if (rc)
+        return rc;
+    CsrViews v{d[0].beg.p, (const int32_t *)d[0].adj.p, d[1].beg.p, (const int32_t *)d[1].adj.p,
+               d[2].beg.p, (const int32_t *)d[2].adj.p};
+    HIPCHK(hipEventRecord(ctx.ev[0], ctx.stream));
+    rc = off64 ? enqueue_count<int64_t>(&ctx, v, t[0].vertices, false, p, L, (TcCtl *)ctx.ctl.p,
+                                        false, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
+                                        ctx.ev[2])
+               : enqueue_count<int32_t>(&ctx, v, t[0].vertices, false, p, L, (TcCtl *)ctx.ctl.p,
+                                        false, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
+                                        ctx.ev[2]);
+    if (rc)
+        return rc;
+    TcCtl h;
+    HIPCHK(hipMemcpyAsync(&h, ctx.ctl.p, sizeof(TcCtl), hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx.ev[0], ctx.ev[2]));
+    for (int q = 0; q < 3; q++) {
+        d[q].beg.release();
+        d[q].adj.release();
+    }
+    out->edge_count = t[2].edges;
+    out->count = h.count;
+    out->time = ms * 1e-3;
+    out->vertices = t[2].vertices - 1;  // With-graph-partition/TC.cu:1152
+    out->degree = t[2].vertices > 0 ? t[2].edges / t[2].vertices : 0;
+    return TRUST_OK;
+}

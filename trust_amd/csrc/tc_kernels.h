@@ -1,0 +1,99 @@
+// Device-side structures and launch wrappers of the counting kernels.
+// Shared by tc_kernels.hip (device code) and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace trust {
+
+// Work tiers.  A vertex u is counted by exactly one tier.
+enum Tier : int { kTierSpill = 0, kTierHub = 1, kTierWave = 2, kTierNone = 3 };
+constexpr int kNumTiers = 3;
+constexpr int kWave = 64;
+constexpr int kMinTableLg = 6;  // smallest table: 64 slots
+constexpr int kPlanPerBlock = 4096;  // vertices per planning workgroup
+
+// Device control block: queue cursors, list bounds and results.  Zeroed by
+// the host (or by a fill kernel under graph capture) before a run.
+struct TcCtl {
+    unsigned long long count;       // triangles
+    unsigned long long probes;      // hash lookups (W)
+    unsigned int next[4];           // queue cursors, one per tier
+    long long list_begin[4];        // tier t occupies list[list_begin[t], list_begin[t+1])
+    unsigned long long cycles;      // instrumentation: summed vertex cycles
+    unsigned long long pad[3];
+};
+
+// Three CSR views: the hash source (N+(u) that is tabled), the walk (the v
+// in N+(u) whose lists are probed) and the probe rows (N+(v)).  For an
+// unpartitioned graph all three are the same arrays; for a 2D tile job they
+// are tiles (i,j), (i,k) and (k,j).
+template <typename OffT>
+struct TcArgs {
+    const OffT *begH;
+    const int32_t *adjH;
+    const OffT *begW;
+    const int32_t *adjW;
+    const OffT *begP;
+    const int32_t *adjP;
+    const int32_t *list;     // vertices grouped by tier (see TcCtl::list_begin)
+    int32_t *spill_mem;      // HBM hash tables, spill_slots ints per workgroup
+    long long spill_slots;
+    int spill_groups;        // workgroups that own an HBM table
+    int table_words;         // LDS ints for tables (hub table = all of it)
+    int hub_lg;              // log2 of the largest hub table that fits LDS
+    int wave_slots;          // LDS ints per wave table
+    int chunk_group;         // vertices per grab, spill/hub tiers
+    int chunk_wave;          // vertices per grab, wave tier
+    int rank;
+    int nranks;
+    TcCtl *ctl;
+    unsigned long long *instr_cycles;  // per vertex (instrumented kernel only)
+    unsigned long long *instr_probes;
+};
+
+// Planner parameters for the degree split.
+template <typename OffT>
+struct PlanArgs {
+    const OffT *begH;
+    const OffT *begW;
+    long long V;
+    int same_csr;       // H, W, P are one CSR: skip d+ < 2 as the reference does
+    int hub_threshold;  // walk degree above which the workgroup tier is used
+    int wave_lg;        // log2 of a wave table
+    int hub_lg;         // log2 of the LDS hub table
+    int force_spill;
+    int wave_only;
+    const unsigned char *tier_override;  // nullable: per-vertex tier chosen elsewhere
+    int32_t *list;
+    long long *block_counts;  // [kNumTiers][nblocks]
+    int per_block;
+    TcCtl *ctl;
+};
+
+// Launch wrappers (tc_kernels.hip).  All launches go on `stream`.
+template <typename OffT>
+hipError_t launch_plan(const PlanArgs<OffT> &p, int nblocks, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
+                        bool instrument, hipStream_t stream);
+template <typename OffT>
+hipError_t count_occupancy(int threads, size_t lds_bytes, bool instrument, int *blocks_per_cu);
+hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
+                                 hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream_t stream);
+
+// Number of LDS ints besides the tables (marks + misc) for `threads`.
+inline int lds_extra_words(int threads) { return (threads / kWave) * kWave + 8; }
+
+// log2 of the table for a hash source of degree d: >= 2d slots, >= 64.
+inline int host_table_lg(long long d)
+{
+    int lg = kMinTableLg;
+    while ((1ll << lg) < 2 * d)
+        lg++;
+    return lg;
+}
+
+}  // namespace trust
