@@ -34,6 +34,7 @@ extern "C" {
 #define TRUST_FLAG_OFFSETS64 2u    /* keep int64 offsets on the device even if E < 2^31 */
 #define TRUST_FLAG_NO_VALIDATE 4u  /* skip the host-side CSR validation */
 #define TRUST_FLAG_WAVE_ONLY 8u    /* no workgroup tier: every vertex that fits a wave table uses one wave */
+#define TRUST_FLAG_LINEAR_PROBING 16u /* linear-probing tables instead of two-choice cuckoo buckets */
 
 /*
  * Launch and schedule parameters.  threads / blocks / chunk / rank /
@@ -151,6 +152,19 @@ int trust_ctx_count(trust_ctx *ctx, const trust_params *p, trust_result *r);
  */
 int trust_ctx_profile_vertices(trust_ctx *ctx, const trust_params *p, uint64_t *cycles,
                                uint64_t *probes, trust_result *r);
+
+/* ---- measurement helpers (no counterpart in the reference) ------------ */
+/*
+ * Gather-only pass: the same plan, tiers and probe walk as trust_ctx_count,
+ * with every hash lookup replaced by a checksum of the loaded ids.  Its
+ * count_seconds is the time of the load path alone, the roofline
+ * denominator for random segment access on this graph (r->triangles holds
+ * the checksum, not a count).
+ */
+int trust_ctx_gather(trust_ctx *ctx, const trust_params *p, trust_result *r);
+/* Device-to-device streaming copy of `bytes`, `reps` times; seconds per copy
+ * (the known-traffic kernel that calibrates HBM byte counters). */
+int trust_membench_copy(int device, int64_t bytes, int reps, double *seconds_per_copy);
 
 /* ---- input contract: files, preprocessing, tiles ---------------------- */
 /* Read <folder>/begin.bin + adjacent.bin (Without-graph-partition/graph.hpp:18-67)

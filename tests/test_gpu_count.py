@@ -76,6 +76,8 @@ def test_rmat16_threads(ctx16, rmat16, threads):
     dict(lds_budget=4096, threads=64, wave_slots=256),
     dict(flags=T.FLAG_FORCE_SPILL), dict(flags=T.FLAG_WAVE_ONLY),
     dict(flags=T.FLAG_FORCE_SPILL, threads=64, blocks=3),
+    dict(flags=T.FLAG_LINEAR_PROBING), dict(flags=T.FLAG_LINEAR_PROBING | T.FLAG_FORCE_SPILL),
+    dict(flags=T.FLAG_LINEAR_PROBING, hub_threshold=0),
 ])
 def test_rmat16_invariance(ctx16, rmat16, kw):
     print(f"[gpu] rmat16 {kw}", flush=True)
@@ -124,7 +126,8 @@ def test_random_dense_graphs(gpu, seed):
     dst = rng.integers(0, n, m).astype(np.int32)
     beg, adj = T.preprocess_edges(src, dst)
     want = oracle.count_hash(beg, adj)
-    for kw in ({}, dict(flags=T.FLAG_FORCE_SPILL), dict(threads=64), dict(hub_threshold=8)):
+    for kw in ({}, dict(flags=T.FLAG_FORCE_SPILL), dict(threads=64), dict(hub_threshold=8),
+               dict(flags=T.FLAG_LINEAR_PROBING)):
         assert T.count_csr(beg, adj, device=gpu, **kw).triangles == want
 
 

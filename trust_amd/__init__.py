@@ -26,7 +26,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 TRUST_OK = 0
 ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2, -3, -4, -5, -6, -7)
 SCHED_DEGREE, SCHED_LEARNED = 0, 1
-FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY = 1, 2, 4, 8
+FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
 
 
 class TrustError(RuntimeError):
@@ -92,6 +92,8 @@ SIGNATURES = {
     "trust_ctx_count": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Result)]),
     "trust_ctx_profile_vertices": (C.c_int, [C.c_void_p, C.POINTER(Params), _PU64, _PU64,
                                              C.POINTER(Result)]),
+    "trust_ctx_gather": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Result)]),
+    "trust_membench_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "trust_csr_load": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "trust_csr_save": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "trust_csr_validate": (C.c_int, [_P64, _P32, C.c_int64, C.c_int64]),
@@ -140,6 +142,13 @@ def _i64(a):
 def _i32(a):
     a = np.ascontiguousarray(a, dtype=np.int32)
     return a, a.ctypes.data_as(_P32)
+
+
+def membench_copy(nbytes: int, reps: int = 5, device: int = 0) -> float:
+    """Seconds per device-to-device streaming copy of nbytes."""
+    t = C.c_double(0)
+    _check(lib().trust_membench_copy(device, nbytes, reps, C.byref(t)))
+    return t.value
 
 
 def device_count() -> int:
@@ -338,6 +347,13 @@ class Context:
         p = make_params(**params)
         r = Result()
         _check(lib().trust_ctx_count(self._h, C.byref(p), C.byref(r)))
+        return r
+
+    def gather(self, **params) -> Result:
+        """Gather-only pass (same walk, no hashing): the load-path time."""
+        p = make_params(**params)
+        r = Result()
+        _check(lib().trust_ctx_gather(self._h, C.byref(p), C.byref(r)))
         return r
 
     def profile_vertices(self, **params):

@@ -48,11 +48,20 @@ struct CsrViews {
     const int32_t *adjP;
 };
 
+// Shape of one counting launch.
 struct LaunchShape {
     int threads = 0, blocks = 0;
     int wave_slots = 0, wave_lg = 0;
-    int table_words = 0, hub_lg = 0;
+    int table_words = 0, group_lg = 0;
     size_t lds_bytes = 0;
+};
+
+// The counting launches of one count: the main launch (hub + wave tiers,
+// small LDS, full occupancy) and, when some table is too large for it, a
+// large-LDS launch (big tier) that also runs the HBM-table spill tier.
+struct LaunchPlan {
+    LaunchShape main, big;
+    bool has_big = false;
     int chunk_group = 1, chunk_wave = 1;
     long long spill_slots = 0;
     int spill_groups = 0;
@@ -82,24 +91,24 @@ int upload_csr(Ctx *ctx, DevCsr &g, const int64_t *beg, const int32_t *adj, int6
 int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,
                  uint32_t flags);
 int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_t V, bool off64,
-                   LaunchShape &L);
+                   LaunchPlan &L);
 int check_params(const trust_params &p);
 double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64);
 
 template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
-                  const trust_params &p, const LaunchShape &L, TcCtl *ctl_dev, bool instrument,
+                  const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
                   const unsigned char *tier_override, hipStream_t s, hipEvent_t ev_plan,
                   hipEvent_t ev_count);
 
-int count_resident(Ctx *ctx, const trust_params &p, bool instrument,
+int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
                    trust_result *r);
 
 // Learned scheduler (learned.hip): fills a per-vertex tier array on the
 // device for the resident graph and returns it through *tiers.
-int learned_plan(Ctx *ctx, const trust_params &p, const LaunchShape &L,
+int learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
                  const unsigned char **tiers);
 
 }  // namespace trust

@@ -90,67 +90,139 @@ static int floor_lg(long long x)
     return lg;
 }
 
-int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_t V,
-                   bool off64, LaunchShape &L)
+// LDS words (pow2) a workgroup of `threads` may use for its table without
+// lowering occupancy below 8 waves per SIMD: 160 KB shared by 32 waves.
+static long long fair_table_words(long long lds_bytes, int threads)
 {
-    L.threads = p.threads > 0 ? p.threads : 512;
-    if (L.threads % kWave != 0 || L.threads > 1024)
-        return fail(TRUST_ERR_ARG, "threads=%d: must be a multiple of 64 and at most 1024",
-                    L.threads);
-    const int nwaves = L.threads / kWave;
-    L.wave_slots = p.wave_slots > 0 ? p.wave_slots : 512;
-    if (!is_pow2(L.wave_slots) || L.wave_slots < 64)
-        return fail(TRUST_ERR_ARG, "wave_slots=%d: must be a power of two >= 64", L.wave_slots);
-    L.wave_lg = floor_lg(L.wave_slots);
-    const int extra = lds_extra_words(L.threads);
-    const long long lds_max = (long long)ctx->lds_max;
-    long long budget = p.lds_budget > 0 ? std::min<long long>(p.lds_budget, lds_max)
-                                        : std::min<long long>(64 * 1024 + extra * 4ll, lds_max);
-    long long budget_words = budget / 4 - extra;
-    const long long wave_words = (long long)nwaves * L.wave_slots;
-    if (budget_words < wave_words)
-        return fail(TRUST_ERR_ARG,
-                    "LDS budget %lld bytes cannot hold %d wave tables of %d slots", budget,
-                    nwaves, L.wave_slots);
-    long long need = 1ll << host_table_lg(std::max<int64_t>(max_deg_hash, 1));
-    long long hub_words = std::min<long long>(need, 1ll << floor_lg(budget_words));
-    L.table_words = (int)std::max(wave_words, hub_words);
-    L.hub_lg = floor_lg(L.table_words);
-    L.lds_bytes = (size_t)(L.table_words + extra) * 4;
+    const int nwaves = threads / kWave;
+    const long long share = lds_bytes * nwaves / 32 / 4 - lds_extra_words(threads);
+    return share > 0 ? 1ll << floor_lg(share) : 0;
+}
+
+static int shape_occupancy(Ctx *ctx, LaunchShape &s, bool off64, int blocks_req)
+{
     int per_cu = 0;
-    hipError_t e = off64 ? count_occupancy<int64_t>(L.threads, L.lds_bytes, false, &per_cu)
-                         : count_occupancy<int32_t>(L.threads, L.lds_bytes, false, &per_cu);
+    hipError_t e = off64 ? count_occupancy<int64_t>(s.threads, s.lds_bytes, false, &per_cu)
+                         : count_occupancy<int32_t>(s.threads, s.lds_bytes, false, &per_cu);
     if (e != hipSuccess)
         return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (per_cu < 1)
         return fail(TRUST_ERR_ARG, "launch of %d threads with %zu LDS bytes cannot run",
-                    L.threads, L.lds_bytes);
-    L.blocks = p.blocks > 0 ? p.blocks : per_cu * ctx->num_cus;
+                    s.threads, s.lds_bytes);
+    s.blocks = blocks_req > 0 ? blocks_req : per_cu * ctx->num_cus;
+    return TRUST_OK;
+}
+
+int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_t V,
+                   bool off64, LaunchPlan &L)
+{
+    L = LaunchPlan();
+    LaunchShape &m = L.main;
+    m.threads = p.threads > 0 ? p.threads : 256;
+    if (m.threads % kWave != 0 || m.threads > 1024)
+        return fail(TRUST_ERR_ARG, "threads=%d: must be a multiple of 64 and at most 1024",
+                    m.threads);
+    const int nwaves = m.threads / kWave;
+    m.wave_slots = p.wave_slots > 0 ? p.wave_slots : 512;
+    if (!is_pow2(m.wave_slots) || m.wave_slots < 64)
+        return fail(TRUST_ERR_ARG, "wave_slots=%d: must be a power of two >= 64", m.wave_slots);
+    m.wave_lg = floor_lg(m.wave_slots);
+    const long long lds_max = (long long)ctx->lds_max;
+    const long long budget = p.lds_budget > 0 ? std::min<long long>(p.lds_budget, lds_max) : lds_max;
+    const long long wave_words = (long long)nwaves * m.wave_slots;
+    if (budget / 4 - lds_extra_words(m.threads) < wave_words)
+        return fail(TRUST_ERR_ARG, "LDS budget %lld bytes cannot hold %d wave tables of %d slots",
+                    budget, nwaves, m.wave_slots);
+    // main launch: group table as large as the fair LDS share allows (and
+    // no larger than the biggest 4d table needs)
+    const long long need4 = 1ll << host_table_lg(2 * std::max<int64_t>(max_deg_hash, 1));
+    long long main_words = std::min(need4, p.lds_budget > 0
+                                               ? 1ll << floor_lg(budget / 4 - lds_extra_words(m.threads))
+                                               : fair_table_words(lds_max, m.threads));
+    m.table_words = (int)std::max(wave_words, main_words);
+    m.group_lg = floor_lg(m.table_words);
+    m.lds_bytes = (size_t)(m.table_words + lds_extra_words(m.threads)) * 4;
+    int rc = shape_occupancy(ctx, m, off64, p.blocks);
+    if (rc)
+        return rc;
     if (p.chunk > 0) {
         L.chunk_group = L.chunk_wave = p.chunk;
     } else {
         L.chunk_group = 1;
-        long long waves = (long long)L.blocks * nwaves;
+        long long waves = (long long)m.blocks * nwaves;
         L.chunk_wave = (int)std::max<long long>(1, std::min<long long>(16, V / (waves * 64 + 1)));
     }
-    // HBM tables for vertices whose table exceeds the LDS hub table.
-    const int max_lg = host_table_lg(std::max<int64_t>(max_deg_hash, 1));
-    const bool any_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) || max_lg > L.hub_lg;
-    if (any_spill) {
-        L.spill_slots = 1ll << max_lg;
-        const long long spill_budget = 2ll << 30;  // bytes of HBM for spill tables
-        long long groups = spill_budget / (L.spill_slots * 4);
-        L.spill_groups = (int)std::max<long long>(1, std::min<long long>(groups, L.blocks));
-    } else {
-        L.spill_slots = 0;
-        L.spill_groups = 0;
+    // large-LDS launch for tables the main launch cannot hold, plus the
+    // HBM-table tier for tables no LDS can hold
+    const int need_lg = host_table_lg(std::max<int64_t>(max_deg_hash, 1));
+    const bool force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) != 0;
+    if (need_lg > m.group_lg || force_spill) {
+        L.has_big = true;
+        LaunchShape &b = L.big;
+        b.threads = 1024;
+        b.wave_slots = m.wave_slots;
+        b.wave_lg = m.wave_lg;
+        const long long extra = lds_extra_words(b.threads);
+        const long long cap_half = 1ll << floor_lg(std::max<long long>(budget / 2 / 4 - extra, 64));
+        const long long cap_full = 1ll << floor_lg(std::max<long long>(budget / 4 - extra, 64));
+        // two workgroups per CU when the 2d table fits half the LDS
+        long long words = (1ll << need_lg) <= cap_half ? std::min(cap_half, need4) : cap_full;
+        words = std::min(words, std::max<long long>(need4, 64));
+        b.table_words = (int)words;
+        b.group_lg = floor_lg(b.table_words);
+        b.lds_bytes = (size_t)(b.table_words + extra) * 4;
+        if (b.lds_bytes > (size_t)budget) {  // budget too small for a 1024-thread workgroup
+            b.threads = 64;
+            b.table_words = 64;
+            b.group_lg = 6;
+            b.lds_bytes = (size_t)(64 + lds_extra_words(64)) * 4;
+        }
+        rc = shape_occupancy(ctx, b, off64, 0);
+        if (rc)
+            return rc;
+        if (need_lg > b.group_lg || force_spill) {
+            L.spill_slots = 1ll << (need_lg + 1);  // 4d slots
+            const long long spill_budget = 2ll << 30;  // bytes of HBM for spill tables
+            long long groups = spill_budget / (L.spill_slots * 4);
+            L.spill_groups = (int)std::max<long long>(1, std::min<long long>(groups, b.blocks));
+        }
     }
     return TRUST_OK;
 }
 
 template <typename OffT>
+static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const trust_params &p,
+                      const LaunchPlan &L, const LaunchShape &s, TcCtl *ctl_dev,
+                      unsigned long long *instr_cycles, unsigned long long *instr_probes)
+{
+    memset(&a, 0, sizeof(a));
+    a.begH = (const OffT *)views.begH;
+    a.adjH = views.adjH;
+    a.begW = (const OffT *)views.begW;
+    a.adjW = views.adjW;
+    a.begP = (const OffT *)views.begP;
+    a.adjP = views.adjP;
+    a.list = (const int32_t *)ctx->list.p;
+    a.spill_mem = (int32_t *)ctx->spill.p;
+    a.spill_slots = L.spill_slots;
+    a.spill_groups = L.spill_groups;
+    a.table_words = s.table_words;
+    a.group_lg = s.group_lg;
+    a.wave_slots = s.wave_slots;
+    a.wave_lg = s.wave_lg;
+    a.chunk_group = L.chunk_group;
+    a.chunk_wave = L.chunk_wave;
+    a.rank = p.rank;
+    a.nranks = p.total_process;
+    a.force_linear = (p.flags & TRUST_FLAG_LINEAR_PROBING) ? 1 : 0;
+    a.ctl = ctl_dev;
+    a.instr_cycles = instr_cycles;
+    a.instr_probes = instr_probes;
+}
+
+template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
-                  const trust_params &p, const LaunchShape &L, TcCtl *ctl_dev, bool instrument,
+                  const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
                   const unsigned char *tier_override, hipStream_t s, hipEvent_t ev_plan,
                   hipEvent_t ev_count)
@@ -177,8 +249,9 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
     pa.V = V;
     pa.same_csr = same_csr ? 1 : 0;
     pa.hub_threshold = p.hub_threshold;
-    pa.wave_lg = L.wave_lg;
-    pa.hub_lg = L.hub_lg;
+    pa.wave_lg = L.main.wave_lg;
+    pa.hub_lg = L.main.group_lg;
+    pa.big_lg = L.has_big ? L.big.group_lg : 0;
     pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
     pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
     pa.tier_override = tier_override;
@@ -191,39 +264,29 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         HIPCHK(hipEventRecord(ev_plan, s));
 
     TcArgs<OffT> a;
-    memset(&a, 0, sizeof(a));
-    a.begH = (const OffT *)views.begH;
-    a.adjH = views.adjH;
-    a.begW = (const OffT *)views.begW;
-    a.adjW = views.adjW;
-    a.begP = (const OffT *)views.begP;
-    a.adjP = views.adjP;
-    a.list = (const int32_t *)ctx->list.p;
-    a.spill_mem = (int32_t *)ctx->spill.p;
-    a.spill_slots = L.spill_slots;
-    a.spill_groups = L.spill_groups;
-    a.table_words = L.table_words;
-    a.hub_lg = L.hub_lg;
-    a.wave_slots = L.wave_slots;
-    a.chunk_group = L.chunk_group;
-    a.chunk_wave = L.chunk_wave;
-    a.rank = p.rank;
-    a.nranks = p.total_process;
-    a.ctl = ctl_dev;
-    a.instr_cycles = instr_cycles;
-    a.instr_probes = instr_probes;
-    HIPCHK(launch_count<OffT>(a, L.blocks, L.threads, L.lds_bytes, instrument, s));
+    if (L.has_big) {
+        fill_args(a, ctx, views, p, L, L.big, ctl_dev, instr_cycles, instr_probes);
+        a.spill_tier = L.spill_groups > 0 ? kTierSpill : -1;
+        a.group_tier = kTierBig;
+        a.wave_tier = -1;
+        HIPCHK(launch_count<OffT>(a, L.big.blocks, L.big.threads, L.big.lds_bytes, mode, s));
+    }
+    fill_args(a, ctx, views, p, L, L.main, ctl_dev, instr_cycles, instr_probes);
+    a.spill_tier = -1;
+    a.group_tier = kTierHub;
+    a.wave_tier = kTierWave;
+    HIPCHK(launch_count<OffT>(a, L.main.blocks, L.main.threads, L.main.lds_bytes, mode, s));
     if (ev_count)
         HIPCHK(hipEventRecord(ev_count, s));
     return TRUST_OK;
 }
 
 template int enqueue_count<int32_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
-                                    const LaunchShape &, TcCtl *, bool, unsigned long long *,
+                                    const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, const unsigned char *, hipStream_t,
                                     hipEvent_t, hipEvent_t);
 template int enqueue_count<int64_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
-                                    const LaunchShape &, TcCtl *, bool, unsigned long long *,
+                                    const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, const unsigned char *, hipStream_t,
                                     hipEvent_t, hipEvent_t);
 
@@ -249,7 +312,7 @@ double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
 }
 
 // Count over the context's resident graph (or a caller-provided view set).
-int count_resident(Ctx *ctx, const trust_params &p, bool instrument,
+int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
                    trust_result *r)
 {
@@ -262,13 +325,14 @@ int count_resident(Ctx *ctx, const trust_params &p, bool instrument,
     if (g.max_deg > (int64_t)(INT32_MAX / 64))
         return fail(TRUST_ERR_OVERFLOW, "max out-degree %lld too large for one probe group",
                     (long long)g.max_deg);
-    LaunchShape L;
+    LaunchPlan L;
     rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, L);
     if (rc)
         return rc;
     rc = ctx->ctl.ensure(sizeof(TcCtl));
     if (rc)
         return rc;
+    const bool instrument = mode == kModeInstr;
     if (instrument) {
         rc = ctx->instr_cycles.ensure(sizeof(unsigned long long) * (size_t)std::max<int64_t>(g.V, 1));
         if (!rc)
@@ -289,12 +353,12 @@ int count_resident(Ctx *ctx, const trust_params &p, bool instrument,
                (const int32_t *)g.adj.p};
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     if (g.off64)
-        rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, instrument,
+        rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, override_tiers,
                                     ctx->stream, ctx->ev[1], ctx->ev[2]);
     else
-        rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, instrument,
+        rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, override_tiers,
                                     ctx->stream, ctx->ev[1], ctx->ev[2]);
@@ -328,13 +392,13 @@ int count_resident(Ctx *ctx, const trust_params &p, bool instrument,
         r->h2d_seconds = ctx->h2d_seconds;
         r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
         r->alg_bytes = alg_bytes(g.V, g.E, h.probes, g.off64);
-        r->n_spill = h.list_begin[1] - h.list_begin[0];
-        r->n_hub = h.list_begin[2] - h.list_begin[1];
-        r->n_wave = h.list_begin[3] - h.list_begin[2];
-        r->threads = L.threads;
-        r->blocks = L.blocks;
-        r->hub_slots = 1 << L.hub_lg;
-        r->wave_slots = L.wave_slots;
+        r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
+        r->n_hub = h.list_begin[kTierHub + 1] - h.list_begin[kTierBig];  // big + hub
+        r->n_wave = h.list_begin[kTierWave + 1] - h.list_begin[kTierWave];
+        r->threads = L.main.threads;
+        r->blocks = L.main.blocks;
+        r->hub_slots = 1 << L.main.group_lg;
+        r->wave_slots = L.main.wave_slots;
     }
     ctx->learned_plan_seconds = 0;
     return TRUST_OK;
@@ -448,7 +512,7 @@ extern "C" int trust_ctx_count(trust_ctx *c, const trust_params *p, trust_result
         return fail(TRUST_ERR_ARG, "null argument");
     Ctx *ctx = reinterpret_cast<Ctx *>(c);
     HIPCHK(hipSetDevice(ctx->device));
-    return count_resident(ctx, *p, false, nullptr, nullptr, r);
+    return count_resident(ctx, *p, kModeCount, nullptr, nullptr, r);
 }
 
 extern "C" int trust_ctx_profile_vertices(trust_ctx *c, const trust_params *p,
@@ -458,7 +522,7 @@ extern "C" int trust_ctx_profile_vertices(trust_ctx *c, const trust_params *p,
         return fail(TRUST_ERR_ARG, "null argument");
     Ctx *ctx = reinterpret_cast<Ctx *>(c);
     HIPCHK(hipSetDevice(ctx->device));
-    return count_resident(ctx, *p, true, (unsigned long long *)cycles,
+    return count_resident(ctx, *p, kModeInstr, (unsigned long long *)cycles,
                           (unsigned long long *)probes, r);
 }
 
@@ -474,7 +538,7 @@ extern "C" int trust_count_csr(const int64_t *beg, const int32_t *adj, int64_t V
     rc = load_ctx_csr(&ctx, beg, adj, V, E, p->flags);
     if (rc)
         return rc;
-    return count_resident(&ctx, *p, false, nullptr, nullptr, r);
+    return count_resident(&ctx, *p, kModeCount, nullptr, nullptr, r);
 }
 
 extern "C" int trust_triangle_count(int rank, const char *folder, int total_process, int threads,
@@ -501,7 +565,7 @@ extern "C" int trust_triangle_count(int rank, const char *folder, int total_proc
     if (rc)
         return rc;
     trust_result r;
-    rc = count_resident(&ctx, p, false, nullptr, nullptr, &r);
+    rc = count_resident(&ctx, p, kModeCount, nullptr, nullptr, &r);
     if (rc)
         return rc;
     out->edge_count = g.E;
@@ -509,5 +573,43 @@ extern "C" int trust_triangle_count(int rank, const char *folder, int total_proc
     out->time = r.seconds;
     out->degree = g.V > 0 ? g.E / g.V : 0;
     out->vertices = g.V;
+    return TRUST_OK;
+}
+
+extern "C" int trust_ctx_gather(trust_ctx *c, const trust_params *p, trust_result *r)
+{
+    if (!c || !p)
+        return fail(TRUST_ERR_ARG, "null argument");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    HIPCHK(hipSetDevice(ctx->device));
+    return count_resident(ctx, *p, kModeGather, nullptr, nullptr, r);
+}
+
+extern "C" int trust_membench_copy(int device, int64_t bytes, int reps, double *seconds_per_copy)
+{
+    if (bytes < 16 || reps < 1 || !seconds_per_copy)
+        return fail(TRUST_ERR_ARG, "bad arguments");
+    Ctx ctx;
+    int rc = ctx_init(&ctx, device);
+    if (rc)
+        return rc;
+    const long long n = bytes / 16;
+    DevBuf a, b;
+    rc = a.ensure((size_t)n * 16);
+    if (!rc)
+        rc = b.ensure((size_t)n * 16);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemsetAsync(a.p, 1, (size_t)n * 16, ctx.stream));
+    const int blocks = ctx.num_cus * 8;
+    HIPCHK(launch_copy((const int4 *)a.p, (int4 *)b.p, n, blocks, ctx.stream));  // warm-up
+    HIPCHK(hipEventRecord(ctx.ev[0], ctx.stream));
+    for (int i = 0; i < reps; i++)
+        HIPCHK(launch_copy((const int4 *)a.p, (int4 *)b.p, n, blocks, ctx.stream));
+    HIPCHK(hipEventRecord(ctx.ev[1], ctx.stream));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx.ev[0], ctx.ev[1]));
+    *seconds_per_copy = ms * 1e-3 / reps;
     return TRUST_OK;
 }

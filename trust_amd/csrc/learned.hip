@@ -3,7 +3,7 @@
 
 namespace trust {
 
-int learned_plan(Ctx *ctx, const trust_params &p, const LaunchShape &L,
+int learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
                  const unsigned char **tiers)
 {
     (void)ctx;

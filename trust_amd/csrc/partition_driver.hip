@@ -93,7 +93,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                     if (rc)
                         return rc;
                 }
-    LaunchShape L;
+    LaunchPlan L;
     int rc = resolve_launch(ctx, p, maxd, Vsum, off64, L);
     if (rc)
         return rc;
@@ -206,9 +206,9 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                    slots[sl[2]].g.beg.p, (const int32_t *)slots[sl[2]].g.adj.p};
         const int64_t rows = tiles[cur[0]].vertices;
         TcCtl *ctl = (TcCtl *)ctls.p + job;
-        rc = off64 ? enqueue_count<int64_t>(ctx, v, rows, false, p, L, ctl, false, nullptr,
+        rc = off64 ? enqueue_count<int64_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
                                             nullptr, nullptr, ctx->stream, nullptr, nullptr)
-                   : enqueue_count<int32_t>(ctx, v, rows, false, p, L, ctl, false, nullptr,
+                   : enqueue_count<int32_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
                                             nullptr, nullptr, ctx->stream, nullptr, nullptr);
         if (rc)
             return rc;
@@ -252,10 +252,10 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         r->count_seconds = r->seconds;
         r->gteps = r->seconds > 0 ? (double)Esum / r->seconds / 1e9 : 0;
         r->alg_bytes = alg_bytes(Vsum, Esum, probes, off64);
-        r->threads = L.threads;
-        r->blocks = L.blocks;
-        r->hub_slots = 1 << L.hub_lg;
-        r->wave_slots = L.wave_slots;
+        r->threads = L.main.threads;
+        r->blocks = L.main.blocks;
+        r->hub_slots = 1 << L.main.group_lg;
+        r->wave_slots = L.main.wave_slots;
     }
     return TRUST_OK;
 }
@@ -348,7 +348,7 @@ extern "C" int trust_triangle_count_partition(int rank, const char *folder, int 
             return rc;
         HIPCHK(hipStreamSynchronize(ctx.stream));
     }
-    LaunchShape L;
+    LaunchPlan L;
     rc = resolve_launch(&ctx, p, maxd, t[0].vertices, off64, L);
     if (rc)
         return rc;
@@ -359,10 +359,10 @@ extern "C" int trust_triangle_count_partition(int rank, const char *folder, int 
                d[2].beg.p, (const int32_t *)d[2].adj.p};
     HIPCHK(hipEventRecord(ctx.ev[0], ctx.stream));
     rc = off64 ? enqueue_count<int64_t>(&ctx, v, t[0].vertices, false, p, L, (TcCtl *)ctx.ctl.p,
-                                        false, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
+                                        kModeCount, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
                                         ctx.ev[2])
                : enqueue_count<int32_t>(&ctx, v, t[0].vertices, false, p, L, (TcCtl *)ctx.ctl.p,
-                                        false, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
+                                        kModeCount, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
                                         ctx.ev[2]);
     if (rc)
         return rc;

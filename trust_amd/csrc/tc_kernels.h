@@ -8,21 +8,31 @@
 namespace trust {
 
 // Work tiers.  A vertex u is counted by exactly one tier.
-enum Tier : int { kTierSpill = 0, kTierHub = 1, kTierWave = 2, kTierNone = 3 };
-constexpr int kNumTiers = 3;
+//   spill: whole workgroup, hash table in HBM (table too big for any LDS)
+//   big:   whole workgroup of the large-LDS launch, table in LDS
+//   hub:   whole workgroup of the main launch, table in LDS
+//   wave:  one wavefront, table in its LDS slice
+enum Tier : int { kTierSpill = 0, kTierBig = 1, kTierHub = 2, kTierWave = 3, kTierNone = 4 };
+constexpr int kNumTiers = 4;
 constexpr int kWave = 64;
-constexpr int kMinTableLg = 6;  // smallest table: 64 slots
+constexpr int kMinTableLg = 6;       // smallest table: 64 slots
 constexpr int kPlanPerBlock = 4096;  // vertices per planning workgroup
+constexpr int kMaxChunks = 256;      // 64-neighbour chunks balanced per group vertex
+constexpr int kUnroll = 4;           // probe windows in flight per wave
 
-// Device control block: queue cursors, list bounds and results.  Zeroed by
-// the host (or by a fill kernel under graph capture) before a run.
+// Kernel modes: count triangles, count + per-vertex instrumentation, or
+// gather only (the same loads, every hash lookup replaced by a checksum of
+// the loaded ids: the load path alone, used as the roofline denominator).
+constexpr int kModeCount = 0, kModeInstr = 1, kModeGather = 2;
+
+// Device control block: queue cursors, list bounds and results.  Cleared
+// by a fill kernel before a run.
 struct TcCtl {
     unsigned long long count;       // triangles
     unsigned long long probes;      // hash lookups (W)
-    unsigned int next[4];           // queue cursors, one per tier
-    long long list_begin[4];        // tier t occupies list[list_begin[t], list_begin[t+1])
-    unsigned long long cycles;      // instrumentation: summed vertex cycles
-    unsigned long long pad[3];
+    unsigned int next[8];           // queue cursors, one per tier
+    long long list_begin[8];        // tier t occupies list[list_begin[t], list_begin[t+1])
+    unsigned long long pad[4];
 };
 
 // Three CSR views: the hash source (N+(u) that is tabled), the walk (the v
@@ -41,13 +51,18 @@ struct TcArgs {
     int32_t *spill_mem;      // HBM hash tables, spill_slots ints per workgroup
     long long spill_slots;
     int spill_groups;        // workgroups that own an HBM table
-    int table_words;         // LDS ints for tables (hub table = all of it)
-    int hub_lg;              // log2 of the largest hub table that fits LDS
+    int spill_tier;          // tier counted with HBM tables here, or -1
+    int group_tier;          // tier counted by whole workgroups with the LDS table, or -1
+    int wave_tier;           // tier counted by single waves, or -1
+    int table_words;         // LDS ints for tables (group table = all of it)
+    int group_lg;            // log2 of the group table capacity
     int wave_slots;          // LDS ints per wave table
-    int chunk_group;         // vertices per grab, spill/hub tiers
+    int wave_lg;
+    int chunk_group;         // vertices per grab, spill/group tiers
     int chunk_wave;          // vertices per grab, wave tier
     int rank;
     int nranks;
+    int force_linear;        // 1: linear-probing tables only (no cuckoo)
     TcCtl *ctl;
     unsigned long long *instr_cycles;  // per vertex (instrumented kernel only)
     unsigned long long *instr_probes;
@@ -62,7 +77,8 @@ struct PlanArgs {
     int same_csr;       // H, W, P are one CSR: skip d+ < 2 as the reference does
     int hub_threshold;  // walk degree above which the workgroup tier is used
     int wave_lg;        // log2 of a wave table
-    int hub_lg;         // log2 of the LDS hub table
+    int hub_lg;         // log2 of the main launch's group table
+    int big_lg;         // log2 of the large-LDS launch's table (0: no such launch)
     int force_spill;
     int wave_only;
     const unsigned char *tier_override;  // nullable: per-vertex tier chosen elsewhere
@@ -77,17 +93,18 @@ template <typename OffT>
 hipError_t launch_plan(const PlanArgs<OffT> &p, int nblocks, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
-                        bool instrument, hipStream_t stream);
+                        int mode, hipStream_t stream);
 template <typename OffT>
-hipError_t count_occupancy(int threads, size_t lds_bytes, bool instrument, int *blocks_per_cu);
+hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, int *blocks_per_cu);
+hipError_t launch_copy(const int4 *src, int4 *dst, long long n, int blocks, hipStream_t stream);
 hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
                                  hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream_t stream);
 
-// Number of LDS ints besides the tables (marks + misc) for `threads`.
-inline int lds_extra_words(int threads) { return (threads / kWave) * kWave + 8; }
+// LDS ints besides the tables: per-wave marks, misc words, chunk totals.
+inline int lds_extra_words(int threads) { return (threads / kWave) * kWave + 8 + kMaxChunks + 8; }
 
-// log2 of the table for a hash source of degree d: >= 2d slots, >= 64.
+// log2 of the smallest table for a hash source of degree d: >= 2d slots, >= 64.
 inline int host_table_lg(long long d)
 {
     int lg = kMinTableLg;

@@ -6,21 +6,29 @@
 //
 // Design (MI355X-first, not a translation of the CUDA kernel):
 //  * wave64 throughout.  A vertex is counted by one wavefront (wave tier),
-//    by a whole workgroup with an LDS table (hub tier) or by a whole
-//    workgroup with an HBM table (spill tier) when its table does not fit
-//    LDS.  Tables are open-addressing (linear probing, Fibonacci hash) with
-//    >= 2 slots per key, so they never overflow (the reference drops entries
-//    past 100 per bucket, TC.cu:293-301) and every probe loop ends at an
-//    empty slot.
-//  * The probe walk processes 64 neighbours v of u at a time: lanes load
+//    by a whole workgroup with an LDS table (hub tier in the main launch,
+//    big tier in a large-LDS launch) or by a whole workgroup with an HBM
+//    table (spill tier).  Tables hold >= 2 slots per key, so they never
+//    overflow (the reference drops entries past 100 per bucket,
+//    TC.cu:293-301).  They are bucketed two-choice cuckoo tables: a lookup
+//    is two 8-byte reads and branch-free compares, so no lane's probe chain
+//    stalls its wave (the reference's linear_search scans a bucket,
+//    TC.cu:70-110).  A vertex whose cuckoo insertion fails falls back to
+//    linear probing, whose loops end at an empty slot.
+//  * The probe walk takes 64 neighbours v of u at a time: lanes load
 //    (beg[v], deg v), an inclusive wave scan (DPP row_shr / row_bcast) lays
 //    the 64 lists end to end, and each 64-wide window of that concatenation
-//    is one coalesced load per lane.  The segment owning each lane is found
-//    with a 64-entry LDS mark array and a wave max-scan; a window inside a
-//    single list skips that and reads its base with one readlane.  This
-//    replaces the reference's per-lane skip loop and its 32-lane
-//    __shfl_sync resynchronisation (TC.cu:513-541) and its 16x64 CTA walk
-//    that is only correct at 1024 threads (TC.cu:330-359).
+//    is one coalesced load per lane.  kUnroll windows are in flight per
+//    wave before any hash lookup, so each lane has kUnroll independent
+//    loads outstanding.  The segment owning each lane is found with a
+//    64-entry LDS mark array and a wave max-scan; a window inside a single
+//    list skips that and reads its base with one readlane.  This replaces
+//    the reference's per-lane skip loop with its 32-lane __shfl_sync
+//    resynchronisation (TC.cu:513-541) and its 16x64 CTA walk that is only
+//    correct at 1024 threads (TC.cu:330-359).
+//  * A workgroup vertex splits its probe windows evenly over its waves
+//    (per-chunk window counts, a prefix, then units of windows pulled from
+//    an LDS counter), so no wave idles while another walks a long list.
 //  * Persistent workgroups pull chunks of vertices from per-tier atomic
 //    queues (the reference's G_INDEX counters, TC.cu:405-422,560-577);
 //    logical rank r of P takes chunks c with c % P == r (TC.cu:259,433).
@@ -91,13 +99,98 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
 }
 
 // ---------------------------------------------------------- hash tables
-__device__ __forceinline__ int table_lg(int d)
+__device__ __forceinline__ int ceil_lg(int x)  // smallest lg with 2^lg >= x (x >= 1)
 {
-    // smallest lg >= kMinTableLg with 2^lg >= 2d
-    int need = 2 * d - 1;
-    int lg = need > 0 ? 32 - __clz(need) : 0;
-    return lg < kMinTableLg ? kMinTableLg : lg;
+    return x > 1 ? 32 - __clz(x - 1) : 0;
 }
+
+// Table size for a hash source of degree d in a tier whose capacity is
+// 2^cap_lg: >= 2d slots (the planner only sends vertices whose 2d fits).
+__device__ __forceinline__ int table_lg(int d, int cap_lg)
+{
+    int lg = ceil_lg(2 * d);
+    lg = lg < kMinTableLg ? kMinTableLg : lg;
+    return lg > cap_lg ? cap_lg : lg;
+}
+
+// ---- bucketed two-choice cuckoo table (the lookup path) ----
+// 2^lg int slots = 2^(lg-1) buckets of two slots.  A key lives in bucket
+// b1 = key & mask or b2 = Fibonacci hash of key, so a lookup is exactly two
+// 8-byte reads and branch-free compares: no probe chain whose longest lane
+// would stall the whole wave.  Load factor <= 1/2, far below the ~0.89
+// limit of 2-way, 2-slot cuckoo hashing.
+__device__ __forceinline__ unsigned int bucket1(int key, int lgb)
+{
+    return (unsigned int)key & ((1u << lgb) - 1u);
+}
+__device__ __forceinline__ unsigned int bucket2(int key, int lgb)
+{
+    return ((unsigned int)key * 0x9E3779B1u) >> (32 - lgb);
+}
+
+template <bool kGlobal>
+__device__ __forceinline__ int2 bucket_load(const int *tab, unsigned int b)
+{
+    if constexpr (kGlobal) {
+        unsigned long long v = __hip_atomic_load((const unsigned long long *)tab + b,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_int2((int)(v & 0xffffffffu), (int)(v >> 32));
+    } else {
+        return *((const int2 *)tab + b);
+    }
+}
+
+// Concurrent insertion: claim an empty slot in either bucket, else evict
+// an occupant with an atomic exchange and carry it to its other bucket.
+// Returns true when `kMaxKicks` evictions did not place every key (the
+// caller then rebuilds the table with linear probing).
+constexpr int kMaxKicks = 48;
+__device__ __forceinline__ bool cuckoo_insert(int *tab, int lgb, int key)
+{
+    for (int it = 0; it < kMaxKicks; it++) {
+        const unsigned int c1 = bucket1(key, lgb), c2 = bucket2(key, lgb);
+        int prev = atomicCAS(tab + 2 * c1, kEmpty, key);
+        if (prev == kEmpty || prev == key)
+            return false;
+        prev = atomicCAS(tab + 2 * c1 + 1, kEmpty, key);
+        if (prev == kEmpty || prev == key)
+            return false;
+        prev = atomicCAS(tab + 2 * c2, kEmpty, key);
+        if (prev == kEmpty || prev == key)
+            return false;
+        prev = atomicCAS(tab + 2 * c2 + 1, kEmpty, key);
+        if (prev == kEmpty || prev == key)
+            return false;
+        const unsigned int slot = 2 * ((it & 1) ? c2 : c1) + (((unsigned int)it >> 1) & 1u);
+        key = atomicExch(tab + slot, key);
+        if (key == kEmpty)
+            return false;
+    }
+    return true;
+}
+
+// Membership count of up to kUnroll keys (key < 0: not a probe).
+template <bool kGlobal>
+__device__ __forceinline__ unsigned int cuckoo_count(const int *tab, int lg, const int (&w)[kUnroll])
+{
+    const int lgb = lg - 1;
+    int2 x1[kUnroll], x2[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+        x1[k] = bucket_load<kGlobal>(tab, bucket1(w[k], lgb));
+        x2[k] = bucket_load<kGlobal>(tab, bucket2(w[k], lgb));
+    }
+    unsigned int hits = 0;
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+        const bool in = (x1[k].x == w[k]) | (x1[k].y == w[k]) | (x2[k].x == w[k]) |
+                        (x2[k].y == w[k]);
+        hits += (unsigned int)(in & (w[k] >= 0));
+    }
+    return hits;
+}
+
+// ---- linear probing (fallback when a cuckoo insertion fails) ----
 
 __device__ __forceinline__ unsigned int hash_slot(int key, int lg)
 {
@@ -122,7 +215,6 @@ __device__ __forceinline__ void slot_clear(int *p)
         *p = kEmpty;
 }
 
-template <bool kGlobal>
 __device__ __forceinline__ void table_insert(int *tab, int lg, int key)
 {
     const unsigned int mask = (1u << lg) - 1u;
@@ -135,117 +227,289 @@ __device__ __forceinline__ void table_insert(int *tab, int lg, int key)
     }
 }
 
+// Membership of up to kUnroll keys (key < 0: skip).  First slots of all
+// keys are read together, then each key finishes its probe sequence.
 template <bool kGlobal>
-__device__ __forceinline__ unsigned int table_contains(const int *tab, int lg, int key)
+__device__ __forceinline__ unsigned int linear_count(const int *tab, int lg, const int (&w)[kUnroll])
 {
     const unsigned int mask = (1u << lg) - 1u;
-    unsigned int h = hash_slot(key, lg);
-    for (;;) {
-        int s = slot_load<kGlobal>(tab + h);
-        if (s == key)
-            return 1u;
-        if (s == kEmpty)
-            return 0u;
-        h = (h + 1u) & mask;
+    unsigned int h[kUnroll];
+    int s[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+        h[k] = hash_slot(w[k], lg);
+        s[k] = w[k] >= 0 ? slot_load<kGlobal>(tab + h[k]) : kEmpty;
     }
+    unsigned int hits = 0;
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+        if (w[k] < 0)
+            continue;
+        for (;;) {
+            if (s[k] == w[k]) {
+                hits++;
+                break;
+            }
+            if (s[k] == kEmpty)
+                break;
+            h[k] = (h[k] + 1u) & mask;
+            s[k] = slot_load<kGlobal>(tab + h[k]);
+        }
+    }
+    return hits;
 }
 
 // ---------------------------------------------------------- probe walk
-// Probes every w in N+(v) for the (up to 64) neighbours v held one per
-// lane, against the table of N+(u).  Must be called by all 64 lanes.
-// `mark` is this wave's 64-int LDS scratch and is all-zero on entry and
-// on exit.  Returns this lane's hits; `total` receives the probe count.
-template <typename OffT, bool kGlobal>
-__device__ __forceinline__ unsigned int probe_group(int v, bool valid, const OffT *__restrict__ begP,
-                                                    const int32_t *__restrict__ adjP, const int *tab,
-                                                    int lg, int *mark, int &total)
+// Lane state for a group of up to 64 neighbours v (one per lane): this
+// lane's list is positions [excl, incl) of the concatenation, whose element
+// at position t is adjP[base + t].
+template <typename OffT>
+struct Group {
+    int incl, excl, deg;
+    OffT base;
+    int total;  // uniform: sum of the 64 degrees
+};
+
+template <typename OffT>
+__device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT *__restrict__ begP)
 {
-    const int lane = lane_id();
     OffT s = 0;
     int deg = 0;
     if (valid) {
         s = begP[v];
         deg = (int)(begP[v + 1] - s);
     }
-    const int incl = wave_scan_add(deg);
-    const int excl = incl - deg;
-    total = readlane(incl, 63);
-    const OffT base = s - (OffT)excl;  // element of window position t is adjP[base_j + t]
-    unsigned int hits = 0;
-    int carry = 0;  // segment that owns the first position of the window
-    for (int b = 0; b < total; b += kWave) {
-        OffT bj;
-        if (readlane(incl, carry) >= b + kWave) {
-            bj = readlane(base, carry);  // the window lies inside one list
-        } else {
-            if (deg > 0 && excl >= b && excl < b + kWave)
-                mark[excl - b] = lane;
-            wave_sync();
-            const int m = mark[lane];
-            mark[lane] = 0;
-            wave_sync();
-            const int j = max(wave_scan_max(m), carry);
-            carry = readlane(j, 63);
-            bj = shfl_idx(base, j);
+    Group<OffT> g;
+    g.deg = deg;
+    g.incl = wave_scan_add(deg);
+    g.excl = g.incl - deg;
+    g.total = readlane(g.incl, 63);
+    g.base = s - (OffT)g.excl;
+    return g;
+}
+
+// Probes windows [wbeg, wend) (64 positions each) of a group's
+// concatenation against the table.  All 64 lanes must call it.  `mark` is
+// this wave's 64-int LDS scratch: all-zero on entry and on exit.
+template <typename OffT, bool kGlobal, int kMode = kModeCount>
+__device__ __forceinline__ unsigned int probe_windows(const Group<OffT> &g, int wbeg, int wend,
+                                                      const int32_t *__restrict__ adjP,
+                                                      const int *tab, int lg, bool linear,
+                                                      int *mark)
+{
+    const int lane = lane_id();
+    const int bend = min(wend * kWave, g.total);
+    int b = wbeg * kWave;
+    if (b >= bend)
+        return 0;
+    // segment holding position b = number of lists ending at or before b
+    int carry = (int)__popcll(__ballot(g.incl <= b));
+    // Issues the loads of the kUnroll windows starting at position `at`
+    // (window positions past bend load nothing and yield -1).
+    auto fetch = [&](int at, int (&w)[kUnroll]) {
+        OffT bj[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            const int bk = at + k * kWave;
+            bj[k] = 0;
+            if (bk >= bend)
+                continue;
+            if (readlane(g.incl, carry) >= bk + kWave) {
+                bj[k] = readlane(g.base, carry);  // the window lies inside one list
+            } else {
+                if (g.deg > 0 && g.excl >= bk && g.excl < bk + kWave)
+                    mark[g.excl - bk] = lane;
+                wave_sync();
+                const int m = mark[lane];
+                mark[lane] = 0;
+                wave_sync();
+                const int j = max(wave_scan_max(m), carry);
+                carry = readlane(j, 63);
+                bj[k] = shfl_idx(g.base, j);
+            }
         }
-        const int t = b + lane;
-        if (t < total)
-            hits += table_contains<kGlobal>(tab, lg, adjP[bj + (OffT)t]);
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) {
+            const int t = at + k * kWave + lane;
+            w[k] = t < bend ? adjP[bj[k] + (OffT)t] : -1;
+        }
+    };
+    // Software pipeline: the next kUnroll windows' loads are in flight while
+    // the current ones are looked up.
+    unsigned int hits = 0;
+    int wcur[kUnroll];
+    fetch(b, wcur);
+    for (;;) {
+        const int bn = b + kWave * kUnroll;
+        const bool more = bn < bend;
+        int wnext[kUnroll];
+        if (more)
+            fetch(bn, wnext);
+        if constexpr (kMode == kModeGather) {
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++)
+                hits += wcur[k] >= 0 ? (unsigned int)wcur[k] : 0u;
+        } else {
+            hits += linear ? linear_count<kGlobal>(tab, lg, wcur)
+                           : cuckoo_count<kGlobal>(tab, lg, wcur);
+        }
+        if (!more)
+            break;
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++)
+            wcur[k] = wnext[k];
+        b = bn;
     }
     return hits;
 }
 
 // ------------------------------------------------------- per-vertex work
-// Whole workgroup counts vertex u with a table at `tab` (LDS or HBM).
-template <typename OffT, bool kGlobal, bool kInstr>
-__device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *tab, int *mark,
-                                             int *cursor, unsigned long long &acc,
-                                             unsigned long long &probes)
+// Whole workgroup counts vertex u with a table at `tab` (LDS or HBM) of
+// capacity 2^cap_lg.  misc[0]: unit cursor, misc[2]: window total;
+// ctot: kMaxChunks+1 ints of LDS.
+template <typename OffT, bool kGlobal, int kMode>
+__device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *tab, int cap_lg,
+                                             int *mark, int *misc, int *ctot,
+                                             unsigned long long &acc, unsigned long long &probes)
 {
+    constexpr bool kInstr = kMode == kModeInstr;
+    constexpr bool kTable = kMode != kModeGather;
     unsigned long long t0 = 0;
     if (kInstr && threadIdx.x == 0)
         t0 = clock64();
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int nwaves = blockDim.x / kWave;
     const OffT sH = a.begH[u];
     const int dH = (int)(a.begH[u + 1] - sH);
     const OffT sW = a.begW[u];
     const int dW = (int)(a.begW[u + 1] - sW);
-    const int lg = table_lg(dH);
+    const int lg = table_lg(dH, cap_lg);
     const int size = 1 << lg;
-    for (int i = threadIdx.x; i < size; i += blockDim.x)
-        slot_clear<kGlobal>(tab + i);
-    if (threadIdx.x == 0)
-        *cursor = 0;
+    if constexpr (kTable)
+        for (int i = threadIdx.x; i < size; i += blockDim.x)
+            slot_clear<kGlobal>(tab + i);
+    if (threadIdx.x == 0) {
+        misc[0] = 0;                 // window-unit cursor
+        misc[3] = a.force_linear;    // linear-probing table for this vertex
+    }
     if constexpr (kGlobal)
         __threadfence();  // HBM slots: cleared values visible in L2 before any insert/lookup
     __syncthreads();
-    for (int i = threadIdx.x; i < dH; i += blockDim.x)
-        table_insert<kGlobal>(tab, lg, a.adjH[sH + (OffT)i]);
+    bool linear = false;
+    if constexpr (kTable) {
+        bool failed = false;
+        if (!a.force_linear)
+            for (int i = threadIdx.x; i < dH; i += blockDim.x)
+                failed |= cuckoo_insert(tab, lg - 1, a.adjH[sH + (OffT)i]);
+        if (failed)
+            misc[3] = 1;  // benign race: every writer stores 1
+        if constexpr (kGlobal)
+            __threadfence();
+        __syncthreads();
+        linear = misc[3] != 0;
+        if (linear) {  // rebuild this vertex's table with linear probing
+            __syncthreads();
+            for (int i = threadIdx.x; i < size; i += blockDim.x)
+                slot_clear<kGlobal>(tab + i);
+            if constexpr (kGlobal)
+                __threadfence();
+            __syncthreads();
+            for (int i = threadIdx.x; i < dH; i += blockDim.x)
+                table_insert(tab, lg, a.adjH[sH + (OffT)i]);
+        }
+    }
     if constexpr (kGlobal)
         __threadfence();
     __syncthreads();
     const int nchunks = (dW + kWave - 1) / kWave;
     unsigned int hits = 0;
     unsigned long long wave_probes = 0;
-    for (;;) {
-        int c = 0;
-        if (lane_id() == 0)
-            c = atomicAdd(cursor, 1);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= nchunks)
-            break;
-        const int idx = c * kWave + lane_id();
-        const bool valid = idx < dW;
-        const int v = valid ? a.adjW[sW + (OffT)idx] : 0;
-        int total;
-        hits += probe_group<OffT, kGlobal>(v, valid, a.begP, a.adjP, tab, lg, mark, total);
-        wave_probes += (unsigned long long)total;
+    if (nchunks <= kMaxChunks) {
+        // window count of every 64-neighbour chunk, then an exclusive prefix
+        for (int c = wave; c < nchunks; c += nwaves) {
+            const int idx = c * kWave + lane;
+            const bool valid = idx < dW;
+            const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
+            if (lane == 0)
+                ctot[c] = (g.total + kWave - 1) / kWave;
+            wave_probes += (unsigned long long)g.total;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            int run = 0;
+            for (int c0 = 0; c0 < nchunks; c0 += kWave) {
+                const int x = c0 + lane < nchunks ? ctot[c0 + lane] : 0;
+                const int inc = wave_scan_add(x);
+                if (c0 + lane < nchunks)
+                    ctot[c0 + lane] = run + inc - x;
+                run += readlane(inc, 63);
+            }
+            if (lane == 0) {
+                ctot[nchunks] = run;
+                misc[2] = run;
+            }
+        }
+        __syncthreads();
+        const int TW = misc[2];
+        // units of K windows, K a multiple of kUnroll
+        int K = TW / (nwaves * 4);
+        K = K < kUnroll ? kUnroll : (K > 64 ? 64 : K);
+        K = (K + kUnroll - 1) / kUnroll * kUnroll;
+        int cur = -1;
+        Group<OffT> g = {0, 0, 0, 0, 0};
+        for (;;) {
+            int k = 0;
+            if (lane == 0)
+                k = atomicAdd(&misc[0], 1);
+            k = __builtin_amdgcn_readfirstlane(k);
+            int w0 = k * K;
+            if (w0 >= TW)
+                break;
+            const int w1 = min(w0 + K, TW);
+            while (w0 < w1) {
+                int lo = 0, hi = nchunks;  // ctot[lo] <= w0 < ctot[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (ctot[mid] <= w0)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                if (lo != cur) {
+                    const int idx = lo * kWave + lane;
+                    const bool valid = idx < dW;
+                    g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
+                    cur = lo;
+                }
+                const int cend = min(w1, ctot[lo + 1]);
+                hits += probe_windows<OffT, kGlobal, kMode>(g, w0 - ctot[lo], cend - ctot[lo], a.adjP,
+                                                     tab, lg, linear, mark);
+                w0 = cend;
+            }
+        }
+    } else {
+        // very long walk lists: whole chunks pulled from the cursor
+        for (;;) {
+            int c = 0;
+            if (lane == 0)
+                c = atomicAdd(&misc[0], 1);
+            c = __builtin_amdgcn_readfirstlane(c);
+            if (c >= nchunks)
+                break;
+            const int idx = c * kWave + lane;
+            const bool valid = idx < dW;
+            const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
+            wave_probes += (unsigned long long)g.total;
+            hits += probe_windows<OffT, kGlobal, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab,
+                                                 lg, linear, mark);
+        }
     }
     acc += hits;
     probes += wave_probes;
     __syncthreads();
     if constexpr (kInstr) {
-        // per-vertex probe total: every wave adds its share
-        if (lane_id() == 0 && wave_probes)
+        if (lane == 0 && wave_probes)
             atomicAdd(a.instr_probes + u, wave_probes);
         if (threadIdx.x == 0)
             a.instr_cycles[u] = clock64() - t0;
@@ -253,10 +517,12 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
 }
 
 // One wavefront counts vertex u with its own LDS table slice.
-template <typename OffT, bool kInstr>
+template <typename OffT, int kMode>
 __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *tab, int *mark,
                                             unsigned long long &acc, unsigned long long &probes)
 {
+    constexpr bool kInstr = kMode == kModeInstr;
+    constexpr bool kTable = kMode != kModeGather;
     unsigned long long t0 = 0;
     if (kInstr)
         t0 = clock64();
@@ -265,22 +531,36 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     const int dH = (int)(a.begH[u + 1] - sH);
     const OffT sW = a.begW[u];
     const int dW = (int)(a.begW[u + 1] - sW);
-    const int lg = table_lg(dH);
+    const int lg = table_lg(dH, a.wave_lg);
     const int size = 1 << lg;
-    for (int i = lane; i < size; i += kWave)
-        tab[i] = kEmpty;
-    wave_sync();
-    for (int i = lane; i < dH; i += kWave)
-        table_insert<false>(tab, lg, a.adjH[sH + (OffT)i]);
-    wave_sync();
+    bool linear = false;
+    if constexpr (kTable) {
+        for (int i = lane; i < size; i += kWave)
+            tab[i] = kEmpty;
+        wave_sync();
+        bool failed = a.force_linear != 0;
+        if (!a.force_linear)
+            for (int i = lane; i < dH; i += kWave)
+                failed |= cuckoo_insert(tab, lg - 1, a.adjH[sH + (OffT)i]);
+        wave_sync();
+        linear = __ballot(failed) != 0;
+        if (linear) {  // rebuild this vertex's table with linear probing
+            for (int i = lane; i < size; i += kWave)
+                tab[i] = kEmpty;
+            wave_sync();
+            for (int i = lane; i < dH; i += kWave)
+                table_insert(tab, lg, a.adjH[sH + (OffT)i]);
+            wave_sync();
+        }
+    }
     unsigned int hits = 0;
     unsigned long long wave_probes = 0;
     for (int c = 0; c < dW; c += kWave) {
         const bool valid = c + lane < dW;
-        const int v = valid ? a.adjW[sW + (OffT)(c + lane)] : 0;
-        int total;
-        hits += probe_group<OffT, false>(v, valid, a.begP, a.adjP, tab, lg, mark, total);
-        wave_probes += (unsigned long long)total;
+        const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)(c + lane)] : 0, valid, a.begP);
+        wave_probes += (unsigned long long)g.total;
+        hits += probe_windows<OffT, false, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab, lg,
+                                           linear, mark);
     }
     wave_sync();
     acc += hits;
@@ -294,7 +574,31 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
 }
 
 // ------------------------------------------------------------ the kernel
-template <typename OffT, bool kInstr>
+template <typename OffT, int kMode, bool kGlobal>
+__device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int *tab, int cap_lg,
+                                            int *mark, int *misc, int *ctot,
+                                            unsigned long long &acc, unsigned long long &probes)
+{
+    const long long lb = a.ctl->list_begin[tier], le = a.ctl->list_begin[tier + 1];
+    if (le <= lb)
+        return;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            misc[1] = (int)atomicAdd(&a.ctl->next[tier], 1u);
+        __syncthreads();
+        const long long c = (long long)(unsigned int)misc[1] * a.nranks + a.rank;
+        const long long first = lb + c * a.chunk_group;
+        if (first >= le)
+            break;
+        const long long last = min(first + (long long)a.chunk_group, le);
+        for (long long i = first; i < last; i++)
+            group_vertex<OffT, kGlobal, kMode>(a, a.list[i], tab, cap_lg, mark, misc, ctot, acc,
+                                                probes);
+    }
+}
+
+template <typename OffT, int kMode>
 __global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
 {
     extern __shared__ int lds[];
@@ -302,60 +606,39 @@ __global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
     int *mark = lds + a.table_words + wave * kWave;
-    int *misc = lds + a.table_words + nwaves * kWave;  // [0] chunk cursor, [1] grab
+    int *misc = lds + a.table_words + nwaves * kWave;  // 8 words
+    int *ctot = misc + 8;                               // kMaxChunks + 1 words
     mark[lane] = 0;
     unsigned long long acc = 0, probes = 0;
-    const long long lb0 = a.ctl->list_begin[0], lb1 = a.ctl->list_begin[1];
-    const long long lb2 = a.ctl->list_begin[2], lb3 = a.ctl->list_begin[3];
 
-    // Spill tier: tables in HBM, one region per participating workgroup.
-    if ((int)blockIdx.x < a.spill_groups && lb1 > lb0) {
+    if (a.spill_tier >= 0 && (int)blockIdx.x < a.spill_groups) {
         int *tab = a.spill_mem + (long long)blockIdx.x * a.spill_slots;
-        for (;;) {
-            __syncthreads();
-            if (threadIdx.x == 0)
-                misc[1] = (int)atomicAdd(&a.ctl->next[kTierSpill], 1u);
-            __syncthreads();
-            const long long c = (long long)(unsigned int)misc[1] * a.nranks + a.rank;
-            const long long first = lb0 + c * a.chunk_group;
-            if (first >= lb1)
-                break;
-            const long long last = min(first + (long long)a.chunk_group, lb1);
-            for (long long i = first; i < last; i++)
-                group_vertex<OffT, true, kInstr>(a, a.list[i], tab, mark, misc, acc, probes);
-        }
+        int cap_lg = 0;
+        while ((2ll << cap_lg) <= a.spill_slots)
+            cap_lg++;
+        group_phase<OffT, kMode, true>(a, a.spill_tier, tab, cap_lg, mark, misc, ctot, acc, probes);
     }
-    // Hub tier: the whole LDS table region is one table.
-    if (lb2 > lb1) {
-        for (;;) {
-            __syncthreads();
-            if (threadIdx.x == 0)
-                misc[1] = (int)atomicAdd(&a.ctl->next[kTierHub], 1u);
-            __syncthreads();
-            const long long c = (long long)(unsigned int)misc[1] * a.nranks + a.rank;
-            const long long first = lb1 + c * a.chunk_group;
-            if (first >= lb2)
-                break;
-            const long long last = min(first + (long long)a.chunk_group, lb2);
-            for (long long i = first; i < last; i++)
-                group_vertex<OffT, false, kInstr>(a, a.list[i], lds, mark, misc, acc, probes);
-        }
-    }
+    if (a.group_tier >= 0)
+        group_phase<OffT, kMode, false>(a, a.group_tier, lds, a.group_lg, mark, misc, ctot, acc,
+                                         probes);
     __syncthreads();
-    // Wave tier: each wave owns a slice of the table region.
-    int *wtab = lds + wave * a.wave_slots;
-    for (;;) {
-        unsigned int k = 0;
-        if (lane == 0)
-            k = atomicAdd(&a.ctl->next[kTierWave], 1u);
-        k = __builtin_amdgcn_readfirstlane(k);
-        const long long c = (long long)k * a.nranks + a.rank;
-        const long long first = lb2 + c * a.chunk_wave;
-        if (first >= lb3)
-            break;
-        const long long last = min(first + (long long)a.chunk_wave, lb3);
-        for (long long i = first; i < last; i++)
-            wave_vertex<OffT, kInstr>(a, a.list[i], wtab, mark, acc, probes);
+    if (a.wave_tier >= 0) {
+        const long long lb = a.ctl->list_begin[a.wave_tier];
+        const long long le = a.ctl->list_begin[a.wave_tier + 1];
+        int *wtab = lds + wave * a.wave_slots;
+        for (;;) {
+            unsigned int k = 0;
+            if (lane == 0)
+                k = atomicAdd(&a.ctl->next[a.wave_tier], 1u);
+            k = __builtin_amdgcn_readfirstlane(k);
+            const long long c = (long long)k * a.nranks + a.rank;
+            const long long first = lb + c * a.chunk_wave;
+            if (first >= le)
+                break;
+            const long long last = min(first + (long long)a.chunk_wave, le);
+            for (long long i = first; i < last; i++)
+                wave_vertex<OffT, kMode>(a, a.list[i], wtab, mark, acc, probes);
+        }
     }
     acc = wave_sum(acc);
     probes = wave_sum(probes) / kWave;  // every lane carried the wave's probe total
@@ -380,8 +663,10 @@ __device__ __forceinline__ int plan_tier(const PlanArgs<OffT> &p, long long u)
     int lg = kMinTableLg;
     while ((1ll << lg) < 2 * dH)
         lg++;
-    if (p.force_spill || lg > p.hub_lg)
+    if (p.force_spill || lg > max(p.hub_lg, p.big_lg))
         return kTierSpill;
+    if (lg > p.hub_lg)
+        return kTierBig;
     if (!p.wave_only && dW > p.hub_threshold)
         return kTierHub;
     if (lg > p.wave_lg)
@@ -392,24 +677,25 @@ __device__ __forceinline__ int plan_tier(const PlanArgs<OffT> &p, long long u)
 template <typename OffT>
 __global__ void __launch_bounds__(256) plan_count_kernel(PlanArgs<OffT> p, int nblocks)
 {
-    __shared__ long long cnt[kNumTiers];
+    __shared__ unsigned long long cnt[kNumTiers];
     if (threadIdx.x < kNumTiers)
         cnt[threadIdx.x] = 0;
     __syncthreads();
     const long long lo = (long long)blockIdx.x * p.per_block;
     const long long hi = min(lo + p.per_block, p.V);
-    int local[kNumTiers] = {0, 0, 0};
+    int local[kNumTiers] = {0, 0, 0, 0};
     for (long long u = lo + threadIdx.x; u < hi; u += blockDim.x) {
         int t = plan_tier(p, u);
-        if (t < kNumTiers)
-            local[t]++;
+#pragma unroll
+        for (int q = 0; q < kNumTiers; q++)
+            local[q] += t == q;
     }
     for (int t = 0; t < kNumTiers; t++)
         if (local[t])
-            atomicAdd((unsigned long long *)&cnt[t], (unsigned long long)local[t]);
+            atomicAdd(&cnt[t], (unsigned long long)local[t]);
     __syncthreads();
     if (threadIdx.x < kNumTiers)
-        p.block_counts[(long long)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+        p.block_counts[(long long)threadIdx.x * nblocks + blockIdx.x] = (long long)cnt[threadIdx.x];
 }
 
 // One workgroup: exclusive scan of the per-block counts of each tier into
@@ -464,6 +750,7 @@ __global__ void __launch_bounds__(256) plan_scatter_kernel(PlanArgs<OffT> p, int
         const long long u = u0 + threadIdx.x;
         const int tier = u < hi ? plan_tier(p, u) : kTierNone;
         int rank_in_wave[kNumTiers];
+#pragma unroll
         for (int t = 0; t < kNumTiers; t++) {
             unsigned long long m = __ballot(tier == t);
             rank_in_wave[t] = __builtin_amdgcn_mbcnt_hi((unsigned int)(m >> 32),
@@ -496,6 +783,22 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
         dst[i] = (int32_t)src[i];
 }
 
+// Plain streaming copy: the known-traffic kernel that calibrates HBM byte
+// counters and gives the sequential-bandwidth reference.
+__global__ void __launch_bounds__(256) copy_kernel(const int4 *__restrict__ src,
+                                                   int4 *__restrict__ dst, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const int4 *src, int4 *dst, long long n, int blocks, hipStream_t stream)
+{
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
 __global__ void fill_u32_kernel(uint32_t *dst, uint32_t value, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -519,26 +822,31 @@ hipError_t launch_plan(const PlanArgs<OffT> &p, int nblocks, hipStream_t stream)
 
 template <typename OffT>
 hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
-                        bool instrument, hipStream_t stream)
+                        int mode, hipStream_t stream)
 {
-    if (instrument)
-        hipLaunchKernelGGL((tc_count_kernel<OffT, true>), dim3(blocks), dim3(threads), lds_bytes,
-                           stream, a);
+    if (mode == kModeInstr)
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeInstr>), dim3(blocks), dim3(threads),
+                           lds_bytes, stream, a);
+    else if (mode == kModeGather)
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeGather>), dim3(blocks), dim3(threads),
+                           lds_bytes, stream, a);
     else
-        hipLaunchKernelGGL((tc_count_kernel<OffT, false>), dim3(blocks), dim3(threads), lds_bytes,
-                           stream, a);
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeCount>), dim3(blocks), dim3(threads),
+                           lds_bytes, stream, a);
     return hipGetLastError();
 }
 
 template <typename OffT>
-hipError_t count_occupancy(int threads, size_t lds_bytes, bool instrument, int *blocks_per_cu)
+hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, int *blocks_per_cu)
 {
-    if (instrument)
+    if (mode == kModeInstr)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, tc_count_kernel<OffT, true>, threads, lds_bytes);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
-                                                        tc_count_kernel<OffT, false>, threads,
-                                                        lds_bytes);
+            blocks_per_cu, tc_count_kernel<OffT, kModeInstr>, threads, lds_bytes);
+    if (mode == kModeGather)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, tc_count_kernel<OffT, kModeGather>, threads, lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, tc_count_kernel<OffT, kModeCount>, threads, lds_bytes);
 }
 
 hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n, hipStream_t stream)
@@ -561,9 +869,9 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream
 
 template hipError_t launch_plan<int32_t>(const PlanArgs<int32_t> &, int, hipStream_t);
 template hipError_t launch_plan<int64_t>(const PlanArgs<int64_t> &, int, hipStream_t);
-template hipError_t launch_count<int32_t>(const TcArgs<int32_t> &, int, int, size_t, bool, hipStream_t);
-template hipError_t launch_count<int64_t>(const TcArgs<int64_t> &, int, int, size_t, bool, hipStream_t);
-template hipError_t count_occupancy<int32_t>(int, size_t, bool, int *);
-template hipError_t count_occupancy<int64_t>(int, size_t, bool, int *);
+template hipError_t launch_count<int32_t>(const TcArgs<int32_t> &, int, int, size_t, int, hipStream_t);
+template hipError_t launch_count<int64_t>(const TcArgs<int64_t> &, int, int, size_t, int, hipStream_t);
+template hipError_t count_occupancy<int32_t>(int, size_t, int, int *);
+template hipError_t count_occupancy<int64_t>(int, size_t, int, int *);
 
 }  // namespace trust
