@@ -77,6 +77,8 @@ typedef struct trust_result {
     int32_t blocks;
     int32_t hub_slots;     /* LDS hub-table slots per workgroup */
     int32_t wave_slots;    /* LDS table slots per wavefront */
+    int32_t schedule_used; /* TRUST_SCHED_* that ran (LEARNED falls back to DEGREE without a model) */
+    int32_t reserved0;
 } trust_result;
 
 /* Library-allocated CSR (trust_csr_free releases it). */
@@ -152,6 +154,40 @@ int trust_ctx_count(trust_ctx *ctx, const trust_params *p, trust_result *r);
  */
 int trust_ctx_profile_vertices(trust_ctx *ctx, const trust_params *p, uint64_t *cycles,
                                uint64_t *probes, trust_result *r);
+
+/* ---- learned cost scheduler (no counterpart in the reference) ---------- */
+/*
+ * Cost model: an MLP 3 -> H -> H -> 1 (ReLU), H = 32, fp32, predicting
+ * log2(1 + single-wave cycles)/32 of a vertex from x = log2(1 + f)/32 of the
+ * features f = (d+(u), sum_{v in N+(u)} d+(v), max_{v in N+(u)} d+(v)).
+ * Parameter layout: W1[H][3], b1[H], W2[H][H], b2[H], W3[H], b3  (H^2+6H+1).
+ * Forward, backward and Adam run as HIP kernels on the device.
+ */
+int64_t trust_mlp_param_count(int hidden);
+int trust_mlp_init(int hidden, uint64_t seed, float *params);
+/* y[n] = MLP(x[n][3]) */
+int trust_mlp_forward(int device, int hidden, const float *params, const float *x, int64_t n,
+                      float *y);
+/* grad[P] of mean((y - target)^2); *loss (nullable) = that mean. */
+int trust_mlp_loss_grad(int device, int hidden, const float *params, const float *x,
+                        const float *target, int64_t n, float *grad, double *loss);
+/* Minibatch Adam (batch, lr), seeded shuffling; params updated in place. */
+int trust_mlp_train(int device, int hidden, float *params, const float *x, const float *target,
+                    int64_t n, int epochs, int batch, float lr, uint64_t seed, double *loss_out);
+/* Scaled features of every vertex of the resident graph, feat[V][3]. */
+int trust_ctx_features(trust_ctx *ctx, float *feat);
+/*
+ * Trains the context's cost model: an instrumented wave-only count records
+ * per-vertex cycles, then the MLP is trained on a seeded sample of at most
+ * max_samples vertices.  *loss: final MSE; *seconds: device time spent.
+ * Counts with schedule TRUST_SCHED_LEARNED then use the model.
+ */
+int trust_ctx_train_cost_model(trust_ctx *ctx, const trust_params *p, int epochs,
+                               int64_t max_samples, uint64_t seed, double *loss, double *seconds);
+int trust_ctx_set_cost_model(trust_ctx *ctx, int hidden, const float *params);
+int trust_ctx_get_cost_model(trust_ctx *ctx, int *hidden, float *params);
+/* Predicted single-wave cycles per vertex (0 for vertices that are not counted). */
+int trust_ctx_predict_costs(trust_ctx *ctx, float *cost);
 
 /* ---- measurement helpers (no counterpart in the reference) ------------ */
 /*

@@ -144,3 +144,61 @@ def ref_tool(name: str) -> str | None:
     """Path of a reference tool built into oracle/_ref (None when absent)."""
     p = os.path.join(REF_DIR, name)
     return p if os.path.exists(p) else None
+
+
+# ---------------------------------------------------------------- cost-model MLP
+# float64 restatement of the network of trust_amd/csrc/mlp.hip (no reference
+# counterpart): x (3) -> relu(W1 x + b1) (H) -> relu(W2 h1 + b2) (H) -> W3 h2 + b3,
+# parameters laid out W1[H][3], b1[H], W2[H][H], b2[H], W3[H], b3.  Pinned
+# by central finite differences in tests/test_oracle.py.
+MLP_H = 32
+
+
+def mlp_unpack(p, H=MLP_H):
+    p = np.asarray(p, np.float64)
+    o = 0
+    W1 = p[o:o + 3 * H].reshape(H, 3); o += 3 * H
+    b1 = p[o:o + H]; o += H
+    W2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    W3 = p[o:o + H]; o += H
+    b3 = p[o]
+    return W1, b1, W2, b2, W3, b3
+
+
+def mlp_forward(p, x, H=MLP_H):
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, H)
+    x = np.asarray(x, np.float64)
+    h1 = np.maximum(x @ W1.T + b1, 0.0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0.0)
+    return h2 @ W3 + b3
+
+
+def mlp_loss_grad(p, x, t, H=MLP_H):
+    """(gradient of mean((y-t)^2) w.r.t. the flat parameters, loss)."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, H)
+    x = np.asarray(x, np.float64)
+    t = np.asarray(t, np.float64)
+    n = len(x)
+    a1 = x @ W1.T + b1
+    h1 = np.maximum(a1, 0.0)
+    a2 = h1 @ W2.T + b2
+    h2 = np.maximum(a2, 0.0)
+    y = h2 @ W3 + b3
+    err = y - t
+    dy = 2.0 * err / n
+    gW3 = h2.T @ dy
+    gb3 = dy.sum()
+    dh2 = np.outer(dy, W3) * (a2 > 0)
+    gW2 = dh2.T @ h1
+    gb2 = dh2.sum(0)
+    dh1 = (dh2 @ W2) * (a1 > 0)
+    gW1 = dh1.T @ x
+    gb1 = dh1.sum(0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3, [gb3]])
+    return g, float((err ** 2).mean())
+
+
+def mlp_scaled_features(beg, adj):
+    """log2(1 + (d+, sum d+(v), max d+(v))) / 32, as the GPU feature kernel."""
+    return (np.log2(1.0 + features(beg, adj)) / 32.0).astype(np.float64)

@@ -1,0 +1,137 @@
+"""GPU tests of the learned cost scheduler: the MLP kernels against the float64
+oracle, GPU training, the feature kernel, and counts under the learned
+schedule (which must equal the degree split's, bit for bit)."""
+import numpy as np
+import pytest
+
+import oracle
+import trust_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def test_param_count_and_init():
+    P = T.mlp_param_count()
+    assert P == 32 * 32 + 6 * 32 + 1
+    a, b = T.mlp_init(7), T.mlp_init(7)
+    assert np.array_equal(a, b) and not np.array_equal(a, T.mlp_init(8))
+
+
+def test_forward_matches_oracle(gpu):
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.3, T.mlp_param_count()).astype(np.float32)
+    x = rng.uniform(0, 1, (5000, 3)).astype(np.float32)
+    y = T.mlp_forward(p, x, device=gpu)
+    want = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.allclose(y, want, rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 300, 4096])
+def test_gradient_matches_oracle(gpu, n):
+    rng = np.random.default_rng(n)
+    p = rng.normal(0, 0.3, T.mlp_param_count()).astype(np.float32)
+    x = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    t = rng.uniform(0, 1, n).astype(np.float32)
+    g, loss = T.mlp_loss_grad(p, x, t, device=gpu)
+    gw, lw = oracle.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    scale = np.abs(gw).max()
+    assert np.max(np.abs(g - gw)) <= 1e-4 * scale + 1e-6
+    assert np.isclose(loss, lw, rtol=1e-4)
+
+
+def test_gradient_deterministic(gpu):
+    rng = np.random.default_rng(9)
+    p = rng.normal(0, 0.3, T.mlp_param_count()).astype(np.float32)
+    x = rng.uniform(0, 1, (3000, 3)).astype(np.float32)
+    t = rng.uniform(0, 1, 3000).astype(np.float32)
+    g1, _ = T.mlp_loss_grad(p, x, t, device=gpu)
+    g2, _ = T.mlp_loss_grad(p, x, t, device=gpu)
+    assert np.array_equal(g1, g2)
+
+
+def test_training_reduces_loss(gpu):
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 1, (20000, 3)).astype(np.float32)
+    t = (0.2 + 0.5 * x[:, 1] + 0.1 * np.maximum(x[:, 0] - 0.3, 0) ** 2).astype(np.float32)
+    p0 = T.mlp_init(1)
+    _, l0 = oracle.mlp_loss_grad(p0.astype(np.float64), x, t)
+    p1, l1 = T.mlp_train(p0, x, t, epochs=15, batch=1024, lr=3e-3, device=gpu)
+    assert l1 < l0 / 20
+    assert np.isclose(l1, oracle.mlp_loss_grad(p1.astype(np.float64), x, t)[1], rtol=1e-3)
+
+
+@pytest.fixture(scope="module")
+def rmat16():
+    beg, adj = T.rmat_csr(16, 16, seed=1)
+    return beg, adj, oracle.count_hash(beg, adj)
+
+
+def test_features_match_oracle(gpu, rmat16):
+    beg, adj, _ = rmat16
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        f = c.features()
+    want = oracle.mlp_scaled_features(beg, adj)
+    assert np.allclose(f, want, rtol=1e-5, atol=1e-6)
+
+
+def test_learned_without_model_falls_back(gpu, rmat16):
+    beg, adj, want = rmat16
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        r = c.count(schedule=T.SCHED_LEARNED)
+        assert r.triangles == want and r.schedule_used == T.SCHED_DEGREE
+
+
+def test_learned_schedule_counts_and_predicts(gpu, rmat16):
+    beg, adj, want = rmat16
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        loss, sec = c.train_cost_model(epochs=20, max_samples=50000, seed=1)
+        assert loss < 0.01  # MSE of log2(1+cycles)/32
+        for kw in ({}, dict(threads=64), dict(threads=1024), dict(chunk=5),
+                   dict(flags=T.FLAG_FORCE_SPILL), dict(rank=1, total_process=2)):
+            r = c.count(schedule=T.SCHED_LEARNED, **kw)
+            assert r.schedule_used == T.SCHED_LEARNED
+            if "rank" in kw:
+                r0 = c.count(schedule=T.SCHED_LEARNED, rank=0, total_process=2)
+                assert r0.triangles + r.triangles == want
+            else:
+                assert r.triangles == want
+                assert r.probes == oracle.probes(beg, adj)
+        # The model ranks vertices by cost.  Cycle counts of small vertices are
+        # noisy, so the bar is the measurement's own repeatability: two
+        # instrumented runs against each other.
+        c1, _, _ = c.profile_vertices(flags=T.FLAG_WAVE_ONLY)
+        c2, _, _ = c.profile_vertices(flags=T.FLAG_WAVE_ONLY)
+        pred = c.predict_costs()
+        m = (c1 > 0) & (c2 > 0) & (pred > 0)
+        lg = lambda a: np.log2(1 + a[m].astype(np.float64))
+        rho_noise = np.corrcoef(lg(c1), lg(c2))[0, 1]
+        rho = np.corrcoef(lg(pred), lg(c2))[0, 1]
+        heavy = c2[m] > np.quantile(c2[m], 0.9)
+        rho_heavy = np.corrcoef(lg(pred)[heavy], lg(c2)[heavy])[0, 1]
+        print(f"[gpu] cost model: mse {loss:.5f}, log-cycle correlation {rho:.3f} "
+              f"(run-to-run {rho_noise:.3f}; top decile {rho_heavy:.3f})", flush=True)
+        assert rho > 0.5 and rho >= rho_noise - 0.15
+        # set/get round trip drives the same schedule
+        params = c.get_cost_model()
+        with T.Context(gpu) as c2:
+            c2.load(beg, adj)
+            c2.set_cost_model(params)
+            r2 = c2.count(schedule=T.SCHED_LEARNED)
+            assert r2.triangles == want and r2.schedule_used == T.SCHED_LEARNED
+
+
+def test_learned_gain_reported(gpu):
+    # what the trained network changes against the degree split on a
+    # mid-size R-MAT: both exact; times printed for the record
+    beg, adj = T.rmat_csr(20, 16, seed=1)
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        c.train_cost_model(epochs=15, max_samples=200000, seed=2)
+        d = min((c.count() for _ in range(3)), key=lambda r: r.seconds)
+        l = min((c.count(schedule=T.SCHED_LEARNED) for _ in range(3)), key=lambda r: r.seconds)
+        assert d.triangles == l.triangles
+        print(f"[gpu] rmat20 degree {d.seconds * 1e3:.2f} ms, learned {l.seconds * 1e3:.2f} ms "
+              f"(plan {l.plan_seconds * 1e3:.2f} ms), gain {d.seconds / l.seconds:.3f}x", flush=True)

@@ -124,3 +124,33 @@ def test_sample_counter_full_stride_is_total():
     assert p == oracle.probes(beg, adj)
     parts = [oracle.count_hash_sample(beg, adj, f, 4)[0] for f in range(4)]
     assert sum(parts) == c
+
+
+def test_mlp_oracle_gradient_finite_differences():
+    rng = np.random.default_rng(0)
+    P = oracle.MLP_H ** 2 + 6 * oracle.MLP_H + 1
+    p = rng.normal(0, 0.3, P)
+    x = rng.uniform(0, 1, (40, 3))
+    t = rng.uniform(0, 1, 40)
+    g, loss = oracle.mlp_loss_grad(p, x, t)
+    assert np.isclose(loss, ((oracle.mlp_forward(p, x) - t) ** 2).mean())
+    eps = 1e-6
+    for i in rng.choice(P, 60, replace=False):
+        e = np.zeros(P)
+        e[i] = eps
+        num = (oracle.mlp_loss_grad(p + e, x, t)[1] - oracle.mlp_loss_grad(p - e, x, t)[1]) / (2 * eps)
+        assert abs(num - g[i]) <= 1e-6 + 1e-5 * abs(num), i
+
+
+def test_mlp_oracle_linear_regime_closed_form():
+    # all weights zero: y = b3 for every x, so d loss / d b3 = 2 mean(b3 - t)
+    # and every other gradient is zero (ReLU inputs are 0, not > 0)
+    P = oracle.MLP_H ** 2 + 6 * oracle.MLP_H + 1
+    p = np.zeros(P)
+    p[-1] = 0.7
+    x = np.random.default_rng(1).uniform(0, 1, (10, 3))
+    t = np.linspace(0, 1, 10)
+    g, loss = oracle.mlp_loss_grad(p, x, t)
+    assert np.isclose(g[-1], 2 * np.mean(0.7 - t))
+    assert np.all(g[:-1] == 0)
+    assert np.isclose(loss, np.mean((0.7 - t) ** 2))

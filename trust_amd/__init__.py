@@ -52,6 +52,7 @@ class Result(C.Structure):
         ("alg_bytes", C.c_double), ("n_spill", C.c_int64), ("n_hub", C.c_int64),
         ("n_wave", C.c_int64), ("threads", C.c_int32), ("blocks", C.c_int32),
         ("hub_slots", C.c_int32), ("wave_slots", C.c_int32),
+        ("schedule_used", C.c_int32), ("reserved0", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -70,6 +71,7 @@ class Arguments(C.Structure):
 
 _P64, _P32 = C.POINTER(C.c_int64), C.POINTER(C.c_int32)
 _PU64 = C.POINTER(C.c_uint64)
+_PF = C.POINTER(C.c_float)
 
 # name -> (restype, argtypes); every symbol of include/trust.h
 SIGNATURES = {
@@ -93,6 +95,20 @@ SIGNATURES = {
     "trust_ctx_profile_vertices": (C.c_int, [C.c_void_p, C.POINTER(Params), _PU64, _PU64,
                                              C.POINTER(Result)]),
     "trust_ctx_gather": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Result)]),
+    "trust_mlp_param_count": (C.c_int64, [C.c_int]),
+    "trust_mlp_init": (C.c_int, [C.c_int, C.c_uint64, _PF]),
+    "trust_mlp_forward": (C.c_int, [C.c_int, C.c_int, _PF, _PF, C.c_int64, _PF]),
+    "trust_mlp_loss_grad": (C.c_int, [C.c_int, C.c_int, _PF, _PF, _PF, C.c_int64, _PF,
+                                      C.POINTER(C.c_double)]),
+    "trust_mlp_train": (C.c_int, [C.c_int, C.c_int, _PF, _PF, _PF, C.c_int64, C.c_int, C.c_int,
+                                  C.c_float, C.c_uint64, C.POINTER(C.c_double)]),
+    "trust_ctx_features": (C.c_int, [C.c_void_p, _PF]),
+    "trust_ctx_train_cost_model": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, C.c_int64,
+                                             C.c_uint64, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double)]),
+    "trust_ctx_set_cost_model": (C.c_int, [C.c_void_p, C.c_int, _PF]),
+    "trust_ctx_get_cost_model": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _PF]),
+    "trust_ctx_predict_costs": (C.c_int, [C.c_void_p, _PF]),
     "trust_membench_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "trust_csr_load": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "trust_csr_save": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
@@ -356,6 +372,36 @@ class Context:
         _check(lib().trust_ctx_gather(self._h, C.byref(p), C.byref(r)))
         return r
 
+    def features(self) -> np.ndarray:
+        """GPU features, shape (V, 3): log2(1 + (d+, sum d+(v), max d+(v))) / 32."""
+        out = np.zeros((self.V, 3), np.float32)
+        _check(lib().trust_ctx_features(self._h, out.ctypes.data_as(_PF)))
+        return out
+
+    def train_cost_model(self, epochs: int = 30, max_samples: int = 200000, seed: int = 1,
+                         **params):
+        """Instrumented wave-only count + GPU training; returns (mse, seconds)."""
+        p = make_params(**params)
+        loss, sec = C.c_double(0), C.c_double(0)
+        _check(lib().trust_ctx_train_cost_model(self._h, C.byref(p), epochs, max_samples, seed,
+                                                C.byref(loss), C.byref(sec)))
+        return loss.value, sec.value
+
+    def set_cost_model(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        _check(lib().trust_ctx_set_cost_model(self._h, MLP_HIDDEN, params.ctypes.data_as(_PF)))
+
+    def get_cost_model(self) -> np.ndarray:
+        h = C.c_int(0)
+        out = np.zeros(mlp_param_count(), np.float32)
+        _check(lib().trust_ctx_get_cost_model(self._h, C.byref(h), out.ctypes.data_as(_PF)))
+        return out
+
+    def predict_costs(self) -> np.ndarray:
+        out = np.zeros(self.V, np.float32)
+        _check(lib().trust_ctx_predict_costs(self._h, out.ctypes.data_as(_PF)))
+        return out
+
     def profile_vertices(self, **params):
         """Instrumented count: (cycles[V], probes[V], Result)."""
         p = make_params(**params)
@@ -365,3 +411,51 @@ class Context:
         _check(lib().trust_ctx_profile_vertices(self._h, C.byref(p), cyc.ctypes.data_as(_PU64),
                                                 prb.ctypes.data_as(_PU64), C.byref(r)))
         return cyc, prb, r
+
+
+# ---------------------------------------------------------------- cost model
+MLP_HIDDEN = 32
+
+
+def mlp_param_count(hidden: int = MLP_HIDDEN) -> int:
+    return int(lib().trust_mlp_param_count(hidden))
+
+
+def mlp_init(seed: int = 1, hidden: int = MLP_HIDDEN) -> np.ndarray:
+    p = np.zeros(mlp_param_count(hidden), np.float32)
+    _check(lib().trust_mlp_init(hidden, seed, p.ctypes.data_as(_PF)))
+    return p
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, np.float32)
+    return a, a.ctypes.data_as(_PF)
+
+
+def mlp_forward(params, x, device: int = 0) -> np.ndarray:
+    params, pp = _f32(params)
+    x, px = _f32(x)
+    y = np.zeros(len(x), np.float32)
+    _check(lib().trust_mlp_forward(device, MLP_HIDDEN, pp, px, len(x), y.ctypes.data_as(_PF)))
+    return y
+
+
+def mlp_loss_grad(params, x, target, device: int = 0):
+    params, pp = _f32(params)
+    x, px = _f32(x)
+    target, pt = _f32(target)
+    g = np.zeros(len(params), np.float32)
+    loss = C.c_double(0)
+    _check(lib().trust_mlp_loss_grad(device, MLP_HIDDEN, pp, px, pt, len(x),
+                                     g.ctypes.data_as(_PF), C.byref(loss)))
+    return g, loss.value
+
+
+def mlp_train(params, x, target, epochs=20, batch=1024, lr=3e-3, seed=1, device: int = 0):
+    params = np.array(params, np.float32, copy=True)
+    x, px = _f32(x)
+    target, pt = _f32(target)
+    loss = C.c_double(0)
+    _check(lib().trust_mlp_train(device, MLP_HIDDEN, params.ctypes.data_as(_PF), px, pt, len(x),
+                                 epochs, batch, lr, seed, C.byref(loss)))
+    return params, loss.value

@@ -99,16 +99,19 @@ template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  const unsigned char *tier_override, hipStream_t s, hipEvent_t ev_plan,
-                  hipEvent_t ev_count);
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count);
 
 int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
                    trust_result *r);
 
-// Learned scheduler (learned.hip): fills a per-vertex tier array on the
-// device for the resident graph and returns it through *tiers.
-int learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
-                 const unsigned char **tiers);
+// Learned scheduler (learned.hip): enqueues features, MLP inference, tier
+// choice and the longest-first counting sort that fill ctx->list and
+// ctl->list_begin for the resident graph.
+int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
+                         hipStream_t s);
+// Instrumented wave-only pass, GPU features, GPU training of the MLP.
+int train_cost_model(Ctx *ctx, const trust_params &p, int epochs, long long max_samples,
+                     uint64_t seed, double *loss_out, double *seconds_out);
 
 }  // namespace trust

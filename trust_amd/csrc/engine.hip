@@ -224,8 +224,7 @@ template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  const unsigned char *tier_override, hipStream_t s, hipEvent_t ev_plan,
-                  hipEvent_t ev_count)
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count)
 {
     int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
     if (rc)
@@ -240,6 +239,7 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         if (rc)
             return rc;
     }
+    if (!preplanned) {
     HIPCHK(launch_fill_u32((uint32_t *)ctl_dev, 0u, sizeof(TcCtl) / 4, s));
 
     PlanArgs<OffT> pa;
@@ -254,12 +254,13 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
     pa.big_lg = L.has_big ? L.big.group_lg : 0;
     pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
     pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
-    pa.tier_override = tier_override;
+    pa.tier_override = nullptr;
     pa.list = (int32_t *)ctx->list.p;
     pa.block_counts = (long long *)ctx->block_counts.p;
     pa.per_block = per_block;
     pa.ctl = ctl_dev;
     HIPCHK(launch_plan<OffT>(pa, nblocks, s));
+    }
     if (ev_plan)
         HIPCHK(hipEventRecord(ev_plan, s));
 
@@ -283,11 +284,11 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
 
 template int enqueue_count<int32_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
-                                    unsigned long long *, const unsigned char *, hipStream_t,
+                                    unsigned long long *, bool, hipStream_t,
                                     hipEvent_t, hipEvent_t);
 template int enqueue_count<int64_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
-                                    unsigned long long *, const unsigned char *, hipStream_t,
+                                    unsigned long long *, bool, hipStream_t,
                                     hipEvent_t, hipEvent_t);
 
 int check_params(const trust_params &p)
@@ -343,24 +344,25 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
         HIPCHK(hipMemsetAsync(ctx->instr_cycles.p, 0, sizeof(unsigned long long) * g.V, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->instr_probes.p, 0, sizeof(unsigned long long) * g.V, ctx->stream));
     }
-    const unsigned char *override_tiers = nullptr;
-    if (p.schedule == TRUST_SCHED_LEARNED) {
-        rc = learned_plan(ctx, p, L, &override_tiers);
-        if (rc)
-            return rc;
-    }
+    // learned schedule when a cost model is loaded, else the degree split
+    const bool learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready;
     CsrViews v{g.beg.p, (const int32_t *)g.adj.p, g.beg.p, (const int32_t *)g.adj.p, g.beg.p,
                (const int32_t *)g.adj.p};
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (learned) {
+        rc = enqueue_learned_plan(ctx, p, L, (TcCtl *)ctx->ctl.p, ctx->stream);
+        if (rc)
+            return rc;
+    }
     if (g.off64)
         rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
-                                    (unsigned long long *)ctx->instr_probes.p, override_tiers,
+                                    (unsigned long long *)ctx->instr_probes.p, learned,
                                     ctx->stream, ctx->ev[1], ctx->ev[2]);
     else
         rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
-                                    (unsigned long long *)ctx->instr_probes.p, override_tiers,
+                                    (unsigned long long *)ctx->instr_probes.p, learned,
                                     ctx->stream, ctx->ev[1], ctx->ev[2]);
     if (rc)
         return rc;
@@ -386,7 +388,7 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
         r->edges = g.E;
         r->triangles = h.count;
         r->probes = h.probes;
-        r->plan_seconds = t_plan * 1e-3 + ctx->learned_plan_seconds;
+        r->plan_seconds = t_plan * 1e-3;
         r->count_seconds = t_count * 1e-3;
         r->seconds = r->plan_seconds + r->count_seconds;
         r->h2d_seconds = ctx->h2d_seconds;
@@ -399,8 +401,8 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
         r->blocks = L.main.blocks;
         r->hub_slots = 1 << L.main.group_lg;
         r->wave_slots = L.main.wave_slots;
+        r->schedule_used = learned ? TRUST_SCHED_LEARNED : TRUST_SCHED_DEGREE;
     }
-    ctx->learned_plan_seconds = 0;
     return TRUST_OK;
 }
 

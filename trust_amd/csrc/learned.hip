@@ -1,16 +1,553 @@
-// Learned cost scheduler (placeholder until the MLP kernels land).
+// Learned cost scheduler.
+//
+// A small MLP (mlp.hip) predicts, from per-vertex degree features, the
+// cycles one wavefront spends on a vertex; it is trained on the GPU on
+// cycles that the instrumented counting kernel records itself.  At count
+// time (inside the timed region) the scheduler
+//   1. computes the features on the GPU: log2(1+d+(u)), log2(1+sum_v d+(v)),
+//      log2(1+max_v d+(v)) over v in N+(u)  (the second is the probe count),
+//   2. evaluates the MLP per vertex,
+//   3. chooses the tier: tables too large for a wave table go to the
+//      workgroup / large-LDS / HBM tiers exactly as the degree split does
+//      (the LDS-vs-HBM choice is by table size); a vertex that fits a wave
+//      table goes to the workgroup tier only when its predicted cost
+//      exceeds half of one wave's fair share of the total predicted work,
+//   4. orders every tier longest-first: 64 logarithmic cost bins (factor
+//      sqrt 2), a stable counting sort so vertices keep their id order
+//      (and the locality of the reference relabel) inside a bin.
+// Without a trained model the degree-threshold split is used (the
+// reference's plain split, Without-graph-partition/TC.cu:19,632) and the
+// result reports which schedule ran.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "engine.h"
+#include "mlp.h"
 
 namespace trust {
 
-int learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
-                 const unsigned char **tiers)
+constexpr int kBins = 64;
+constexpr int kKeys = kNumTiers * kBins;  // 256
+constexpr unsigned short kKeyNone = 0xFFFF;
+
+__device__ __forceinline__ int lane64() { return (int)__lane_id(); }
+
+__device__ __forceinline__ unsigned long long wsum_u64(unsigned long long x)
 {
-    (void)ctx;
-    (void)p;
-    (void)L;
-    *tiers = nullptr;
-    return fail(TRUST_ERR_ARG, "learned schedule not available in this build");
+    for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_xor(x, off);
+    return x;
+}
+__device__ __forceinline__ int wmax_i(int x)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        x = max(x, __shfl_xor(x, off));
+    return x;
+}
+
+// One wave per vertex: d+(u), sum and max of d+(v), log-scaled.
+template <typename OffT>
+__global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ beg,
+                                                       const int32_t *__restrict__ adj, long long V,
+                                                       float *__restrict__ feat)
+{
+    const long long waves = (long long)gridDim.x * (blockDim.x / 64);
+    const int lane = lane64();
+    for (long long u = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; u < V;
+         u += waves) {
+        const OffT s = beg[u];
+        const int d = (int)(beg[u + 1] - s);
+        unsigned long long sum = 0;
+        int mx = 0;
+        for (int c = 0; c < d; c += 64) {
+            if (c + lane < d) {
+                const int v = adj[s + (OffT)(c + lane)];
+                const int dv = (int)(beg[v + 1] - beg[v]);
+                sum += (unsigned long long)dv;
+                mx = max(mx, dv);
+            }
+        }
+        sum = wsum_u64(sum);
+        mx = wmax_i(mx);
+        if (lane == 0) {
+            feat[3 * u] = log2f(1.f + (float)d) * kFeatScale;
+            feat[3 * u + 1] = log2f(1.f + (float)sum) * kFeatScale;
+            feat[3 * u + 2] = log2f(1.f + (float)mx) * kFeatScale;
+        }
+    }
+}
+
+// Predicted single-wave cycles per vertex (0 for vertices that are not
+// counted); per-block partial sums of the predictions.
+template <typename OffT>
+__global__ void __launch_bounds__(256) cost_kernel(const OffT *__restrict__ beg, long long V,
+                                                   const float *__restrict__ feat,
+                                                   const float *__restrict__ params,
+                                                   float *__restrict__ cost,
+                                                   double *__restrict__ block_sum)
+{
+    constexpr int H = kMlpHidden;
+    __shared__ float p[kMlpParams<H>];
+    __shared__ double part[4];
+    for (int i = threadIdx.x; i < kMlpParams<H>; i += blockDim.x)
+        p[i] = params[i];
+    __syncthreads();
+    const float *W1 = p, *b1 = p + 3 * H, *W2 = p + 4 * H, *b2 = W2 + H * H, *W3 = b2 + H,
+                *b3 = W3 + H;
+    double local = 0;
+    for (long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x; u < V;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int d = (int)(beg[u + 1] - beg[u]);
+        float c = 0.f;
+        if (d >= 2) {
+            const float x0 = feat[3 * u], x1 = feat[3 * u + 1], x2 = feat[3 * u + 2];
+            float h1[H];
+#pragma unroll
+            for (int i = 0; i < H; i++) {
+                float a = b1[i] + W1[3 * i] * x0 + W1[3 * i + 1] * x1 + W1[3 * i + 2] * x2;
+                h1[i] = a > 0.f ? a : 0.f;
+            }
+            float y = b3[0];
+            for (int i = 0; i < H; i++) {
+                float a = b2[i];
+#pragma unroll
+                for (int j = 0; j < H; j++)
+                    a += W2[i * H + j] * h1[j];
+                y += W3[i] * (a > 0.f ? a : 0.f);
+            }
+            y = fminf(fmaxf(y, 0.f), 1.f);  // log2(1+cycles)/32 in [0, 1]
+            c = exp2f(y * 32.f) - 1.f;
+        }
+        cost[u] = c;
+        local += (double)c;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        local += __shfl_xor(local, off);
+    if (lane64() == 0)
+        part[threadIdx.x / 64] = local;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        block_sum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// total = sum of block partial sums (fixed order), one thread.
+__global__ void sum_blocks_kernel(const double *block_sum, int nb, double *total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double a = 0;
+        for (int b = 0; b < nb; b++)
+            a += block_sum[b];
+        *total = a;
+    }
+}
+
+struct KeyArgs {
+    long long V;
+    int same_csr;
+    int wave_lg, hub_lg, big_lg;
+    int force_spill;
+    double split_fraction;  // wave -> workgroup when cost > fraction * total / waves
+    double waves;
+};
+
+template <typename OffT>
+__global__ void __launch_bounds__(256) keys_kernel(const OffT *__restrict__ beg, KeyArgs k,
+                                                   const float *__restrict__ cost,
+                                                   const double *__restrict__ total,
+                                                   unsigned short *__restrict__ keys)
+{
+    const double tau = k.split_fraction * (*total) / k.waves;
+    for (long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x; u < k.V;
+         u += (long long)gridDim.x * blockDim.x) {
+        const long long d = (long long)(beg[u + 1] - beg[u]);
+        unsigned short key = kKeyNone;
+        if (d >= 2) {
+            int lg = kMinTableLg;
+            while ((1ll << lg) < 2 * d)
+                lg++;
+            int tier;
+            if (k.force_spill || lg > max(k.hub_lg, k.big_lg))
+                tier = kTierSpill;
+            else if (lg > k.hub_lg)
+                tier = kTierBig;
+            else if (lg > k.wave_lg)
+                tier = kTierHub;
+            else
+                tier = (double)cost[u] > tau ? kTierHub : kTierWave;
+            int bin = (int)(log2f(1.f + cost[u]) * 2.f);
+            bin = bin < 0 ? 0 : (bin >= kBins ? kBins - 1 : bin);
+            key = (unsigned short)(tier * kBins + (kBins - 1 - bin));
+        }
+        keys[u] = key;
+    }
+}
+
+// ---- stable counting sort of vertices by key into the work list ----
+__global__ void __launch_bounds__(256) key_hist_kernel(const unsigned short *__restrict__ keys,
+                                                       long long V, long long per_block,
+                                                       long long *__restrict__ counts, int nb)
+{
+    __shared__ unsigned int h[kKeys];
+    for (int i = threadIdx.x; i < kKeys; i += blockDim.x)
+        h[i] = 0;
+    __syncthreads();
+    const long long lo = (long long)blockIdx.x * per_block, hi = min(lo + per_block, V);
+    for (long long u = lo + threadIdx.x; u < hi; u += blockDim.x) {
+        const unsigned short key = keys[u];
+        if (key != kKeyNone)
+            atomicAdd(&h[key], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kKeys; i += blockDim.x)
+        counts[(long long)i * nb + blockIdx.x] = h[i];
+}
+
+// Exclusive scan over counts in key-major order; tier t starts at key t*kBins.
+__global__ void __launch_bounds__(1024) key_scan_kernel(long long *counts, int nb, TcCtl *ctl)
+{
+    __shared__ long long part[1024];
+    __shared__ long long base;
+    if (threadIdx.x == 0)
+        base = 0;
+    __syncthreads();
+    const long long n = (long long)kKeys * nb;
+    for (long long c0 = 0; c0 < n; c0 += blockDim.x) {
+        const long long i = c0 + threadIdx.x;
+        const long long x = i < n ? counts[i] : 0;
+        part[threadIdx.x] = x;
+        __syncthreads();
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            const long long y = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += y;
+            __syncthreads();
+        }
+        const long long incl = part[threadIdx.x];
+        if (i < n) {
+            const long long off = base + incl - x;
+            counts[i] = off;
+            const long long key = i / nb, blk = i % nb;
+            if (blk == 0 && key % kBins == 0)
+                ctl->list_begin[key / kBins] = off;
+        }
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1)
+            base += incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        ctl->list_begin[kNumTiers] = base;
+}
+
+__global__ void __launch_bounds__(256) key_scatter_kernel(const unsigned short *__restrict__ keys,
+                                                          long long V, long long per_block,
+                                                          const long long *__restrict__ offsets,
+                                                          int nb, int32_t *__restrict__ list)
+{
+    __shared__ long long run[kKeys];
+    __shared__ unsigned short wcnt[4][kKeys];
+    const int wave = threadIdx.x / 64, lane = lane64();
+    for (int i = threadIdx.x; i < kKeys; i += blockDim.x) {
+        run[i] = offsets[(long long)i * nb + blockIdx.x];
+        wcnt[0][i] = wcnt[1][i] = wcnt[2][i] = wcnt[3][i] = 0;
+    }
+    __syncthreads();
+    const long long lo = (long long)blockIdx.x * per_block, hi = min(lo + per_block, V);
+    for (long long u0 = lo; u0 < hi; u0 += blockDim.x) {
+        const long long u = u0 + threadIdx.x;
+        const int key = u < hi ? (int)keys[u] : (int)kKeyNone;
+        int rank = 0;
+        unsigned long long remaining = __ballot(key != (int)kKeyNone);
+        while (remaining) {
+            const int lead = __ffsll((long long)remaining) - 1;
+            const int kk = __shfl(key, lead);
+            const unsigned long long m = __ballot(key == kk);
+            if (key == kk)
+                rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned int)(m >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0));
+            if (lane == 0)
+                wcnt[wave][kk] = (unsigned short)__popcll(m);
+            remaining &= ~m;
+        }
+        __syncthreads();
+        if (key != (int)kKeyNone) {
+            long long pos = run[key] + rank;
+            for (int w = 0; w < wave; w++)
+                pos += wcnt[w][key];
+            list[pos] = (int32_t)u;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kKeys; i += blockDim.x) {
+            run[i] += (long long)wcnt[0][i] + wcnt[1][i] + wcnt[2][i] + wcnt[3][i];
+            wcnt[0][i] = wcnt[1][i] = wcnt[2][i] = wcnt[3][i] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- host side
+template <typename OffT>
+static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s)
+{
+    const long long V = ctx->g.V;
+    if (V == 0)
+        return TRUST_OK;
+    const int blocks = (int)std::min<long long>((V + 3) / 4, (long long)ctx->num_cus * 32);
+    hipLaunchKernelGGL(features_kernel<OffT>, dim3(blocks), dim3(256), 0, s,
+                       (const OffT *)ctx->g.beg.p, (const int32_t *)ctx->g.adj.p, V, feat);
+    HIPCHK(hipGetLastError());
+    return TRUST_OK;
+}
+
+static int features_dispatch(Ctx *ctx, float *feat, hipStream_t s)
+{
+    return ctx->g.off64 ? enqueue_features<int64_t>(ctx, feat, s)
+                        : enqueue_features<int32_t>(ctx, feat, s);
+}
+
+template <typename OffT>
+static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
+                                  TcCtl *ctl, hipStream_t s)
+{
+    const long long V = ctx->g.V;
+    int rc;
+    if ((rc = ctx->feat.ensure(sizeof(float) * 3 * std::max<long long>(V, 1))) ||
+        (rc = ctx->cost.ensure(sizeof(float) * std::max<long long>(V, 1))) ||
+        (rc = ctx->tiers.ensure(sizeof(unsigned short) * std::max<long long>(V, 1))) ||
+        (rc = ctx->list.ensure(sizeof(int32_t) * std::max<long long>(V, 1))))
+        return rc;
+    HIPCHK(launch_fill_u32((uint32_t *)ctl, 0u, sizeof(TcCtl) / 4, s));
+    rc = enqueue_features<OffT>(ctx, (float *)ctx->feat.p, s);
+    if (rc)
+        return rc;
+    const int cb = (int)std::max<long long>(1, std::min<long long>((V + 255) / 256, 4096));
+    // block sums (cb doubles) + total (1 double) live in block_counts' space
+    const long long per_block = std::max<long long>(4096, (V + 1023) / 1024);
+    const int nb = (int)std::max<long long>(1, (V + per_block - 1) / per_block);
+    if ((rc = ctx->block_counts.ensure(sizeof(long long) * ((size_t)kKeys * nb + cb + 1))))
+        return rc;
+    long long *counts = (long long *)ctx->block_counts.p;
+    double *bsum = (double *)(counts + (size_t)kKeys * nb);
+    double *total = bsum + cb;
+    hipLaunchKernelGGL(cost_kernel<OffT>, dim3(cb), dim3(256), 0, s, (const OffT *)ctx->g.beg.p, V,
+                       (const float *)ctx->feat.p, (const float *)ctx->mlp_params.p,
+                       (float *)ctx->cost.p, bsum);
+    hipLaunchKernelGGL(sum_blocks_kernel, dim3(1), dim3(64), 0, s, bsum, cb, total);
+    KeyArgs k;
+    k.V = V;
+    k.same_csr = 1;
+    k.wave_lg = L.main.wave_lg;
+    k.hub_lg = L.main.group_lg;
+    k.big_lg = L.has_big ? L.big.group_lg : 0;
+    k.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
+    k.split_fraction = 0.5;
+    k.waves = (double)L.main.blocks * (L.main.threads / kWave);
+    hipLaunchKernelGGL(keys_kernel<OffT>, dim3(cb), dim3(256), 0, s, (const OffT *)ctx->g.beg.p, k,
+                       (const float *)ctx->cost.p, (const double *)total,
+                       (unsigned short *)ctx->tiers.p);
+    hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, s,
+                       (const unsigned short *)ctx->tiers.p, V, per_block, counts, nb);
+    hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, s, counts, nb, ctl);
+    hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, s,
+                       (const unsigned short *)ctx->tiers.p, V, per_block, (const long long *)counts,
+                       nb, (int32_t *)ctx->list.p);
+    HIPCHK(hipGetLastError());
+    return TRUST_OK;
+}
+
+int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
+                         hipStream_t s)
+{
+    if (!ctx->mlp_ready)
+        return fail(TRUST_ERR_ARG, "no cost model");
+    return ctx->g.off64 ? enqueue_learned_plan_t<int64_t>(ctx, p, L, ctl, s)
+                        : enqueue_learned_plan_t<int32_t>(ctx, p, L, ctl, s);
+}
+
+// Instrumented wave-only pass -> features -> GPU training.
+int train_cost_model(Ctx *ctx, const trust_params &p_in, int epochs, long long max_samples,
+                     uint64_t seed, double *loss_out, double *seconds_out)
+{
+    const long long V = ctx->g.V;
+    if (V < 1)
+        return fail(TRUST_ERR_ARG, "context holds no graph");
+    trust_params p = p_in;
+    p.schedule = TRUST_SCHED_DEGREE;
+    p.flags |= TRUST_FLAG_WAVE_ONLY;
+    p.rank = 0;
+    p.total_process = 1;
+    std::vector<unsigned long long> cyc((size_t)V);
+    trust_result r;
+    int rc = count_resident(ctx, p, kModeInstr, cyc.data(), nullptr, &r);
+    if (rc)
+        return rc;
+    const double t0 = r.seconds;
+    rc = ctx->feat.ensure(sizeof(float) * 3 * V);
+    if (rc)
+        return rc;
+    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream);
+    if (rc)
+        return rc;
+    std::vector<float> feat((size_t)3 * V);
+    HIPCHK(hipMemcpyAsync(feat.data(), ctx->feat.p, sizeof(float) * 3 * V, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // training set: counted vertices (cycles > 0), a seeded sample
+    std::vector<long long> idx;
+    for (long long u = 0; u < V; u++)
+        if (cyc[u] > 0)
+            idx.push_back(u);
+    if (idx.empty())
+        return fail(TRUST_ERR_ARG, "no counted vertex to train on");
+    uint64_t st = seed ^ 0x5851F42D4C957F2Dull;
+    auto rnd = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    const long long n = std::min<long long>((long long)idx.size(), std::max<long long>(max_samples, 1));
+    for (long long i = 0; i < n; i++)
+        std::swap(idx[i], idx[i + (long long)(rnd() % (uint64_t)(idx.size() - i))]);
+    // vertices too big for a wave table ran on a workgroup: scale to wave-cycles
+    const double wg_waves = r.threads / kWave;
+    std::vector<float> x((size_t)3 * n), t((size_t)n);
+    for (long long i = 0; i < n; i++) {
+        const long long u = idx[i];
+        x[3 * i] = feat[3 * u];
+        x[3 * i + 1] = feat[3 * u + 1];
+        x[3 * i + 2] = feat[3 * u + 2];
+        const long long d = (long long)std::llround(exp2((double)feat[3 * u] / kFeatScale) - 1.0);
+        const bool wave_table = host_table_lg(d) <= 31 && (1ll << host_table_lg(d)) <= r.wave_slots;
+        double c = (double)cyc[u] * (wave_table ? 1.0 : wg_waves);
+        t[i] = (float)(log2(1.0 + c) * kFeatScale);
+    }
+    DevBuf dx, dt;
+    const int P = kMlpParams<kMlpHidden>;
+    if ((rc = dx.ensure(sizeof(float) * 3 * n)) || (rc = dt.ensure(sizeof(float) * n)) ||
+        (rc = ctx->mlp_params.ensure(sizeof(float) * P)))
+        return rc;
+    std::vector<float> init(P);
+    mlp_init_host(seed, init.data());
+    HIPCHK(hipMemcpyAsync(ctx->mlp_params.p, init.data(), sizeof(float) * P, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(dx.p, x.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dt.p, t.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    rc = mlp_train_device(ctx, (float *)ctx->mlp_params.p, (const float *)dx.p, (const float *)dt.p,
+                          n, epochs, 4096, 3e-3f, seed, loss_out);
+    if (rc)
+        return rc;
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[2]));
+    if (seconds_out)
+        *seconds_out = t0 + ms * 1e-3;
+    ctx->mlp_ready = true;
+    return TRUST_OK;
 }
 
 }  // namespace trust
+
+using namespace trust;
+
+extern "C" int trust_ctx_features(trust_ctx *c, float *feat)
+{
+    if (!c || !feat)
+        return fail(TRUST_ERR_ARG, "null argument");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    HIPCHK(hipSetDevice(ctx->device));
+    const long long V = ctx->g.V;
+    int rc = ctx->feat.ensure(sizeof(float) * 3 * std::max<long long>(V, 1));
+    if (rc)
+        return rc;
+    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(feat, ctx->feat.p, sizeof(float) * 3 * V, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return TRUST_OK;
+}
+
+extern "C" int trust_ctx_train_cost_model(trust_ctx *c, const trust_params *p, int epochs,
+                                          int64_t max_samples, uint64_t seed, double *loss,
+                                          double *seconds)
+{
+    if (!c || !p || epochs < 0)
+        return fail(TRUST_ERR_ARG, "bad arguments");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    HIPCHK(hipSetDevice(ctx->device));
+    return train_cost_model(ctx, *p, epochs, max_samples, seed, loss, seconds);
+}
+
+extern "C" int trust_ctx_set_cost_model(trust_ctx *c, int hidden, const float *params)
+{
+    if (!c || !params || hidden != kMlpHidden)
+        return fail(TRUST_ERR_ARG, "bad arguments (hidden must be %d)", kMlpHidden);
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    HIPCHK(hipSetDevice(ctx->device));
+    const int P = kMlpParams<kMlpHidden>;
+    int rc = ctx->mlp_params.ensure(sizeof(float) * P);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpy(ctx->mlp_params.p, params, sizeof(float) * P, hipMemcpyHostToDevice));
+    ctx->mlp_ready = true;
+    return TRUST_OK;
+}
+
+extern "C" int trust_ctx_get_cost_model(trust_ctx *c, int *hidden, float *params)
+{
+    if (!c || !hidden)
+        return fail(TRUST_ERR_ARG, "null argument");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    if (!ctx->mlp_ready)
+        return fail(TRUST_ERR_ARG, "no cost model trained or set");
+    *hidden = kMlpHidden;
+    if (params) {
+        HIPCHK(hipSetDevice(ctx->device));
+        HIPCHK(hipMemcpy(params, ctx->mlp_params.p, sizeof(float) * kMlpParams<kMlpHidden>,
+                         hipMemcpyDeviceToHost));
+    }
+    return TRUST_OK;
+}
+
+extern "C" int trust_ctx_predict_costs(trust_ctx *c, float *cost)
+{
+    if (!c || !cost)
+        return fail(TRUST_ERR_ARG, "null argument");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    if (!ctx->mlp_ready)
+        return fail(TRUST_ERR_ARG, "no cost model trained or set");
+    HIPCHK(hipSetDevice(ctx->device));
+    const long long V = ctx->g.V;
+    int rc;
+    const int cb = (int)std::max<long long>(1, std::min<long long>((V + 255) / 256, 4096));
+    if ((rc = ctx->feat.ensure(sizeof(float) * 3 * std::max<long long>(V, 1))) ||
+        (rc = ctx->cost.ensure(sizeof(float) * std::max<long long>(V, 1))) ||
+        (rc = ctx->block_counts.ensure(sizeof(double) * (cb + 1))))
+        return rc;
+    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream);
+    if (rc)
+        return rc;
+    if (ctx->g.off64)
+        hipLaunchKernelGGL(cost_kernel<int64_t>, dim3(cb), dim3(256), 0, ctx->stream,
+                           (const int64_t *)ctx->g.beg.p, V, (const float *)ctx->feat.p,
+                           (const float *)ctx->mlp_params.p, (float *)ctx->cost.p,
+                           (double *)ctx->block_counts.p);
+    else
+        hipLaunchKernelGGL(cost_kernel<int32_t>, dim3(cb), dim3(256), 0, ctx->stream,
+                           (const int32_t *)ctx->g.beg.p, V, (const float *)ctx->feat.p,
+                           (const float *)ctx->mlp_params.p, (float *)ctx->cost.p,
+                           (double *)ctx->block_counts.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cost, ctx->cost.p, sizeof(float) * V, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return TRUST_OK;
+}

@@ -207,9 +207,9 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         const int64_t rows = tiles[cur[0]].vertices;
         TcCtl *ctl = (TcCtl *)ctls.p + job;
         rc = off64 ? enqueue_count<int64_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
-                                            nullptr, nullptr, ctx->stream, nullptr, nullptr)
+                                            nullptr, false, ctx->stream, nullptr, nullptr)
                    : enqueue_count<int32_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
-                                            nullptr, nullptr, ctx->stream, nullptr, nullptr);
+                                            nullptr, false, ctx->stream, nullptr, nullptr);
         if (rc)
             return rc;
         for (int q = 0; q < 3; q++)
@@ -359,10 +359,10 @@ extern "C" int trust_triangle_count_partition(int rank, const char *folder, int 
                d[2].beg.p, (const int32_t *)d[2].adj.p};
     HIPCHK(hipEventRecord(ctx.ev[0], ctx.stream));
     rc = off64 ? enqueue_count<int64_t>(&ctx, v, t[0].vertices, false, p, L, (TcCtl *)ctx.ctl.p,
-                                        kModeCount, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
+                                        kModeCount, nullptr, nullptr, false, ctx.stream, ctx.ev[1],
                                         ctx.ev[2])
                : enqueue_count<int32_t>(&ctx, v, t[0].vertices, false, p, L, (TcCtl *)ctx.ctl.p,
-                                        kModeCount, nullptr, nullptr, nullptr, ctx.stream, ctx.ev[1],
+                                        kModeCount, nullptr, nullptr, false, ctx.stream, ctx.ev[1],
                                         ctx.ev[2]);
     if (rc)
         return rc;
