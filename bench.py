@@ -12,6 +12,16 @@ The graph: R-MAT scale S, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05),
 seed 1, preprocessed into the reference's degree-oriented CSR (same bytes as
 the reference's Preprocess tools).  Generation, preprocessing and the
 host-to-device copy happen before the timed region.
+
+Three schedules run on the same resident graph and must agree on the count:
+  * the plain degree-threshold split (hub threshold 100) with the default
+    launch shape,
+  * the same split at the reference README's launch (1024 x 1024, chunk 1),
+  * the learned schedule (default headline): the cost model is trained on
+    this graph first (instrumented pass + GPU training, reported separately);
+    its feature, inference and queue-building kernels run inside every timed
+    step.
+`value` is the headline schedule's GTEPS = E / device seconds per step.
 """
 import argparse
 import json
@@ -60,6 +70,19 @@ def cpu_baseline(beg, adj, E, W, target_seconds):
     }
 
 
+def committed_traffic(workload):
+    """HBM bytes per counting launch from the committed rocprof summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, calibrated on the copy kernel), or
+    None when no summary exists for it."""
+    path = os.path.join(ROOT, "profiles", f"tc_count_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,7 +91,7 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--schedule", choices=["degree", "learned"], default="degree")
+    ap.add_argument("--schedule", choices=["degree", "learned"], default="learned")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
@@ -99,36 +122,57 @@ def main():
     h2d = ctx.load(beg, adj)
     log(f"h2d {h2d:.2f}s ({(beg.nbytes + adj.nbytes) / h2d / 1e9:.1f} GB/s incl. pageable staging)")
 
-    sched = T.SCHED_LEARNED if args.schedule == "learned" else T.SCHED_DEGREE
-    kw = dict(schedule=sched, threads=args.threads, blocks=args.blocks, chunk=args.chunk)
-    ref_count = None
-    for i in range(args.warmup):
-        r = ctx.count(**kw)
-        log(f"warmup {i}: {r.triangles} triangles, {r.seconds * 1e3:.1f} ms")
-        if ref_count is None:
-            ref_count = r.triangles
-        if r.triangles != ref_count:
-            raise SystemExit("count changed between runs")
+    base_kw = dict(threads=args.threads, blocks=args.blocks, chunk=args.chunk)
 
-    results = []
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        r = ctx.count(**kw)
-        results.append(r)
-    t_end = time.perf_counter()
-    counts = {r.triangles for r in results}
-    if len(counts) != 1 or (ref_count is not None and counts != {ref_count}):
-        raise SystemExit(f"counts differ across steps: {counts}")
-    r0 = results[0]
-    ms_wall = (t_end - t_start) / args.steps * 1e3
-    dev_s = sum(r.seconds for r in results) / len(results)
-    cnt_s = sum(r.count_seconds for r in results) / len(results)
+    def run(kw, warmup, steps, label):
+        """warmup + timed steps of one schedule; counts must be identical."""
+        ref = None
+        for i in range(warmup):
+            r = ctx.count(**kw)
+            ref = r.triangles if ref is None else ref
+            if r.triangles != ref:
+                raise SystemExit(f"{label}: count changed between runs")
+        res = []
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            res.append(ctx.count(**kw))
+        t_end = time.perf_counter()
+        counts = {r.triangles for r in res} | ({ref} if ref is not None else set())
+        if len(counts) != 1:
+            raise SystemExit(f"{label}: counts differ across steps: {counts}")
+        dev_s = sum(r.seconds for r in res) / len(res)
+        cnt_s = sum(r.count_seconds for r in res) / len(res)
+        log(f"{label}: {res[0].triangles} triangles, {dev_s * 1e3:.1f} ms/step "
+            f"(count kernels {cnt_s * 1e3:.1f} ms, plan {(dev_s - cnt_s) * 1e3:.2f} ms), "
+            f"{E / dev_s / 1e9:.3f} GTEPS")
+        return dict(r=res[0], dev_s=dev_s, cnt_s=cnt_s,
+                    wall_ms=(t_end - t_start) / steps * 1e3)
+
+    # the plain degree-threshold split (hub threshold 100), autotuned launch
+    deg = run(dict(base_kw, schedule=T.SCHED_DEGREE), args.warmup, args.steps, "degree split")
+    # the same split at the reference README's launch (1024 threads x 1024 blocks, chunk 1)
+    ref_launch = run(dict(schedule=T.SCHED_DEGREE, threads=1024, blocks=1024, chunk=1), 1,
+                     max(1, args.steps // 2), "degree split, reference launch")
+    # learned schedule: train the cost model on this graph (timed separately)
+    train = None
+    if args.schedule == "learned":
+        mse, train_s = ctx.train_cost_model(epochs=15, max_samples=200000, seed=1, **base_kw)
+        log(f"cost model trained: mse {mse:.5f} (log-cycle units/32), {train_s:.2f}s device time")
+        train = dict(mse=mse, seconds=train_s)
+        head = run(dict(base_kw, schedule=T.SCHED_LEARNED), args.warmup, args.steps, "learned")
+        if head["r"].schedule_used != T.SCHED_LEARNED:
+            raise SystemExit("learned schedule did not run")
+    else:
+        head = deg
+    if head["r"].triangles != deg["r"].triangles or ref_launch["r"].triangles != deg["r"].triangles:
+        raise SystemExit("schedules disagree on the count")
+    r0, dev_s, cnt_s = head["r"], head["dev_s"], head["cnt_s"]
     W = r0.probes
     achieved = r0.alg_bytes / cnt_s / 1e9
     # Load-path reference on the same graph: the same plan and walk with the
     # hash lookups replaced by a checksum (measured random-segment bandwidth),
     # and a plain streaming copy (measured sequential bandwidth).
-    gkw = dict(kw, schedule=T.SCHED_DEGREE)
+    gkw = dict(base_kw, schedule=T.SCHED_DEGREE)
     ctx.gather(**gkw)
     gs = [ctx.gather(**gkw).count_seconds for _ in range(3)]
     gather_gbs = r0.alg_bytes / (sum(gs) / len(gs)) / 1e9
@@ -137,6 +181,8 @@ def main():
     copy_gbs = 2 * copy_bytes / copy_s / 1e9
     log(f"count kernel {cnt_s * 1e3:.1f} ms ({achieved:.0f} GB/s alg); gather-only "
         f"{sum(gs) / len(gs) * 1e3:.1f} ms ({gather_gbs:.0f} GB/s); copy {copy_gbs:.0f} GB/s")
+    workload = f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}"
+    traffic = committed_traffic(workload)
     line = {
         "metric": METRIC,
         "value": E / dev_s / 1e9,
@@ -145,20 +191,25 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": dev_s * 1e3,
-        "ms_per_step_wall": ms_wall,
+        "ms_per_step_wall": head["wall_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
         "config": {
-            "workload": f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}",
+            "workload": workload,
             "generator": "R-MAT a=0.57 b=0.19 c=0.19 d=0.05, reference preprocess contract",
             "V": V, "E": E, "W_probes": W, "triangles": r0.triangles,
             "schedule": args.schedule, "threads": r0.threads, "blocks": r0.blocks,
             "tiers_spill_hub_wave": [r0.n_spill, r0.n_hub, r0.n_wave],
             "parallelism": "single-gpu",
             "count_kernel_ms": cnt_s * 1e3, "plan_ms": (dev_s - cnt_s) * 1e3,
+            "degree_split_gteps": E / deg["dev_s"] / 1e9,
+            "degree_split_reference_launch_gteps": E / ref_launch["dev_s"] / 1e9,
+            "speedup_vs_degree_split": deg["dev_s"] / dev_s,
+            "speedup_vs_degree_split_reference_launch": ref_launch["dev_s"] / dev_s,
+            "cost_model_training": train,
             "h2d_s": h2d, "preprocess_s": t2 - t1,
         },
         "roofline": {
@@ -167,7 +218,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "alg_bytes_per_launch": r0.alg_bytes,
             "gather_only_GBs": gather_gbs,
             "frac_of_gather_only": achieved / gather_gbs,
@@ -175,8 +226,6 @@ def main():
         },
     }
     if not args.no_cpu:
-        import oracle
-
         line["cpu_baseline"] = cpu_baseline(beg, adj, E, W, args.cpu_seconds)
     ctx.close()
     print(json.dumps(line), flush=True)

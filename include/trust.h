@@ -35,6 +35,7 @@ extern "C" {
 #define TRUST_FLAG_NO_VALIDATE 4u  /* skip the host-side CSR validation */
 #define TRUST_FLAG_WAVE_ONLY 8u    /* no workgroup tier: every vertex that fits a wave table uses one wave */
 #define TRUST_FLAG_LINEAR_PROBING 16u /* linear-probing tables instead of two-choice cuckoo buckets */
+#define TRUST_FLAG_XCD_SPLIT 32u   /* 8 jobs by v % 8, job x first on XCD x (per-XCD L2 locality) */
 
 /*
  * Launch and schedule parameters.  threads / blocks / chunk / rank /

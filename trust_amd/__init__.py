@@ -27,6 +27,7 @@ TRUST_OK = 0
 ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2, -3, -4, -5, -6, -7)
 SCHED_DEGREE, SCHED_LEARNED = 0, 1
 FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
+FLAG_XCD_SPLIT = 32
 
 
 class TrustError(RuntimeError):

@@ -48,6 +48,16 @@ struct CsrViews {
     const int32_t *adjP;
 };
 
+// Walk views of a multi-job count (XCD class split); row u of job j is
+// adjW[begW[u], endW[u]).
+struct JobSet {
+    int n = 0;
+    bool xcd = false;
+    const void *begW[kMaxJobs];
+    const void *endW[kMaxJobs];
+    const int32_t *adjW[kMaxJobs];
+};
+
 // Shape of one counting launch.
 struct LaunchShape {
     int threads = 0, blocks = 0;
@@ -79,6 +89,9 @@ struct Ctx {
     double h2d_seconds = 0;
     double learned_plan_seconds = 0;
     DevBuf staging, list, block_counts, ctl, spill, instr_cycles, instr_probes;
+    // XCD class split: rows of g.adj grouped by v % class_n, class starts
+    DevBuf class_starts;
+    int class_n = 0;
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     bool mlp_ready = false;
@@ -99,7 +112,8 @@ template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count);
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
+                  const JobSet *jobs = nullptr);
 
 int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,

@@ -199,10 +199,13 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.begH = (const OffT *)views.begH;
     a.adjH = views.adjH;
     a.begW = (const OffT *)views.begW;
+    a.endW = (const OffT *)views.begW + 1;
     a.adjW = views.adjW;
     a.begP = (const OffT *)views.begP;
     a.adjP = views.adjP;
     a.list = (const int32_t *)ctx->list.p;
+    a.njobs = 1;
+    a.jobs[0] = TcJob<OffT>{a.begW, a.endW, a.adjW, a.list, ctl_dev};
     a.spill_mem = (int32_t *)ctx->spill.p;
     a.spill_slots = L.spill_slots;
     a.spill_groups = L.spill_groups;
@@ -224,11 +227,20 @@ template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count)
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
+                  const JobSet *jobs)
 {
-    int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
+    const int nj = jobs ? jobs->n : 1;
+    int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1) * nj);
     if (rc)
         return rc;
+    const OffT *jbeg[kMaxJobs], *jend[kMaxJobs];
+    const int32_t *jadj[kMaxJobs];
+    for (int j = 0; j < nj; j++) {
+        jbeg[j] = (const OffT *)(jobs ? jobs->begW[j] : views.begW);
+        jend[j] = jobs ? (const OffT *)jobs->endW[j] : (const OffT *)views.begW + 1;
+        jadj[j] = jobs ? jobs->adjW[j] : views.adjW;
+    }
     const int per_block = kPlanPerBlock;
     const int nblocks = (int)((V + per_block - 1) / per_block);
     rc = ctx->block_counts.ensure(sizeof(long long) * (size_t)kNumTiers * std::max(nblocks, 1));
@@ -239,13 +251,15 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         if (rc)
             return rc;
     }
-    if (!preplanned) {
-    HIPCHK(launch_fill_u32((uint32_t *)ctl_dev, 0u, sizeof(TcCtl) / 4, s));
+    if (!preplanned)
+    for (int j = 0; j < nj; j++) {
+    HIPCHK(launch_fill_u32((uint32_t *)(ctl_dev + j), 0u, sizeof(TcCtl) / 4, s));
 
     PlanArgs<OffT> pa;
     memset(&pa, 0, sizeof(pa));
     pa.begH = (const OffT *)views.begH;
-    pa.begW = (const OffT *)views.begW;
+    pa.begW = jbeg[j];
+    pa.endW = jend[j];
     pa.V = V;
     pa.same_csr = same_csr ? 1 : 0;
     pa.hub_threshold = p.hub_threshold;
@@ -255,24 +269,34 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
     pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
     pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
     pa.tier_override = nullptr;
-    pa.list = (int32_t *)ctx->list.p;
+    pa.list = (int32_t *)ctx->list.p + (size_t)j * std::max<int64_t>(V, 1);
     pa.block_counts = (long long *)ctx->block_counts.p;
     pa.per_block = per_block;
-    pa.ctl = ctl_dev;
+    pa.ctl = ctl_dev + j;
     HIPCHK(launch_plan<OffT>(pa, nblocks, s));
     }
     if (ev_plan)
         HIPCHK(hipEventRecord(ev_plan, s));
 
     TcArgs<OffT> a;
+    auto set_jobs = [&](TcArgs<OffT> &x) {
+        x.njobs = nj;
+        x.xcd_jobs = jobs && jobs->xcd ? 1 : 0;
+        for (int j = 0; j < nj; j++)
+            x.jobs[j] = TcJob<OffT>{jbeg[j], jend[j], jadj[j],
+                                    (const int32_t *)ctx->list.p + (size_t)j * std::max<int64_t>(V, 1),
+                                    ctl_dev + j};
+    };
     if (L.has_big) {
         fill_args(a, ctx, views, p, L, L.big, ctl_dev, instr_cycles, instr_probes);
+        set_jobs(a);
         a.spill_tier = L.spill_groups > 0 ? kTierSpill : -1;
         a.group_tier = kTierBig;
         a.wave_tier = -1;
         HIPCHK(launch_count<OffT>(a, L.big.blocks, L.big.threads, L.big.lds_bytes, mode, s));
     }
     fill_args(a, ctx, views, p, L, L.main, ctl_dev, instr_cycles, instr_probes);
+    set_jobs(a);
     a.spill_tier = -1;
     a.group_tier = kTierHub;
     a.wave_tier = kTierWave;
@@ -285,11 +309,11 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
 template int enqueue_count<int32_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t);
+                                    hipEvent_t, hipEvent_t, const JobSet *);
 template int enqueue_count<int64_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t);
+                                    hipEvent_t, hipEvent_t, const JobSet *);
 
 int check_params(const trust_params &p)
 {
@@ -312,6 +336,37 @@ double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
     return ob * (double)(V + 1) + 4.0 * (double)E + 2.0 * ob * (double)E + 4.0 * (double)W;
 }
 
+// Rows of the resident graph grouped by v % C, with per-class row starts
+// (the XCD class split).  Any row order is a valid oriented CSR, so the
+// regrouped graph serves every later count.
+static int ensure_class_layout(Ctx *ctx, int C)
+{
+    DevCsr &g = ctx->g;
+    if (ctx->class_n == C)
+        return TRUST_OK;
+    const size_t ob = g.off64 ? 8 : 4;
+    DevBuf out;
+    int rc = out.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(g.E, 1));
+    if (!rc)
+        rc = ctx->class_starts.ensure(ob * (size_t)(C - 1) * (size_t)std::max<int64_t>(g.V, 1));
+    if (rc)
+        return rc;
+    const int blocks = ctx->num_cus * 16;
+    if (g.off64)
+        HIPCHK(launch_class_layout<int64_t>((const int64_t *)g.beg.p, (const int32_t *)g.adj.p,
+                                            (int32_t *)out.p, g.V, C,
+                                            (int64_t *)ctx->class_starts.p, blocks, ctx->stream));
+    else
+        HIPCHK(launch_class_layout<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
+                                            (int32_t *)out.p, g.V, C,
+                                            (int32_t *)ctx->class_starts.p, blocks, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::swap(g.adj.p, out.p);
+    std::swap(g.adj.bytes, out.bytes);
+    ctx->class_n = C;
+    return TRUST_OK;
+}
+
 // Count over the context's resident graph (or a caller-provided view set).
 int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
@@ -330,7 +385,28 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
     rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, L);
     if (rc)
         return rc;
-    rc = ctx->ctl.ensure(sizeof(TcCtl));
+    // XCD class split: job x walks the v with v % 8 == x, on XCD x first
+    JobSet js;
+    const JobSet *jsp = nullptr;
+    if ((p.flags & TRUST_FLAG_XCD_SPLIT) && g.V > 0) {
+        const int C = kMaxJobs;
+        rc = ensure_class_layout(ctx, C);
+        if (rc)
+            return rc;
+        const size_t ob = g.off64 ? 8 : 4;
+        const char *starts = (const char *)ctx->class_starts.p;
+        js.n = C;
+        js.xcd = true;
+        for (int x = 0; x < C; x++) {
+            js.begW[x] = x == 0 ? g.beg.p : starts + ob * (size_t)(x - 1) * g.V;
+            js.endW[x] = x == C - 1 ? (const void *)((const char *)g.beg.p + ob)
+                                    : starts + ob * (size_t)x * g.V;
+            js.adjW[x] = (const int32_t *)g.adj.p;
+        }
+        jsp = &js;
+    }
+    const int nj = jsp ? jsp->n : 1;
+    rc = ctx->ctl.ensure(sizeof(TcCtl) * nj);
     if (rc)
         return rc;
     const bool instrument = mode == kModeInstr;
@@ -345,7 +421,8 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
         HIPCHK(hipMemsetAsync(ctx->instr_probes.p, 0, sizeof(unsigned long long) * g.V, ctx->stream));
     }
     // learned schedule when a cost model is loaded, else the degree split
-    const bool learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready;
+    // (the learned queue is single-job: with the XCD split it is not used)
+    const bool learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready && !jsp;
     CsrViews v{g.beg.p, (const int32_t *)g.adj.p, g.beg.p, (const int32_t *)g.adj.p, g.beg.p,
                (const int32_t *)g.adj.p};
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -358,16 +435,17 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
         rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, learned,
-                                    ctx->stream, ctx->ev[1], ctx->ev[2]);
+                                    ctx->stream, ctx->ev[1], ctx->ev[2], jsp);
     else
         rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, learned,
-                                    ctx->stream, ctx->ev[1], ctx->ev[2]);
+                                    ctx->stream, ctx->ev[1], ctx->ev[2], jsp);
     if (rc)
         return rc;
-    TcCtl h;
-    HIPCHK(hipMemcpyAsync(&h, ctx->ctl.p, sizeof(TcCtl), hipMemcpyDeviceToHost, ctx->stream));
+    TcCtl hj[kMaxJobs];
+    HIPCHK(hipMemcpyAsync(hj, ctx->ctl.p, sizeof(TcCtl) * nj, hipMemcpyDeviceToHost, ctx->stream));
+    TcCtl &h = hj[0];
     if (instrument) {
         if (cycles_out)
             HIPCHK(hipMemcpyAsync(cycles_out, ctx->instr_cycles.p,
@@ -394,9 +472,11 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
         r->h2d_seconds = ctx->h2d_seconds;
         r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
         r->alg_bytes = alg_bytes(g.V, g.E, h.probes, g.off64);
-        r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
-        r->n_hub = h.list_begin[kTierHub + 1] - h.list_begin[kTierBig];  // big + hub
-        r->n_wave = h.list_begin[kTierWave + 1] - h.list_begin[kTierWave];
+        for (int j = 0; j < nj; j++) {  // vertex-job pairs per tier
+            r->n_spill += hj[j].list_begin[kTierSpill + 1] - hj[j].list_begin[kTierSpill];
+            r->n_hub += hj[j].list_begin[kTierHub + 1] - hj[j].list_begin[kTierBig];  // big + hub
+            r->n_wave += hj[j].list_begin[kTierWave + 1] - hj[j].list_begin[kTierWave];
+        }
         r->threads = L.main.threads;
         r->blocks = L.main.blocks;
         r->hub_slots = 1 << L.main.group_lg;

@@ -383,7 +383,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
     const OffT sH = a.begH[u];
     const int dH = (int)(a.begH[u + 1] - sH);
     const OffT sW = a.begW[u];
-    const int dW = (int)(a.begW[u + 1] - sW);
+    const int dW = (int)(a.endW[u] - sW);
     const int lg = table_lg(dH, cap_lg);
     const int size = 1 << lg;
     if constexpr (kTable)
@@ -530,7 +530,7 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     const OffT sH = a.begH[u];
     const int dH = (int)(a.begH[u + 1] - sH);
     const OffT sW = a.begW[u];
-    const int dW = (int)(a.begW[u + 1] - sW);
+    const int dW = (int)(a.endW[u] - sW);
     const int lg = table_lg(dH, a.wave_lg);
     const int size = 1 << lg;
     bool linear = false;
@@ -611,42 +611,59 @@ __global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
     mark[lane] = 0;
     unsigned long long acc = 0, probes = 0;
 
-    if (a.spill_tier >= 0 && (int)blockIdx.x < a.spill_groups) {
-        int *tab = a.spill_mem + (long long)blockIdx.x * a.spill_slots;
-        int cap_lg = 0;
-        while ((2ll << cap_lg) <= a.spill_slots)
-            cap_lg++;
-        group_phase<OffT, kMode, true>(a, a.spill_tier, tab, cap_lg, mark, misc, ctot, acc, probes);
-    }
-    if (a.group_tier >= 0)
-        group_phase<OffT, kMode, false>(a, a.group_tier, lds, a.group_lg, mark, misc, ctot, acc,
-                                         probes);
-    __syncthreads();
-    if (a.wave_tier >= 0) {
-        const long long lb = a.ctl->list_begin[a.wave_tier];
-        const long long le = a.ctl->list_begin[a.wave_tier + 1];
-        int *wtab = lds + wave * a.wave_slots;
-        for (;;) {
-            unsigned int k = 0;
-            if (lane == 0)
-                k = atomicAdd(&a.ctl->next[a.wave_tier], 1u);
-            k = __builtin_amdgcn_readfirstlane(k);
-            const long long c = (long long)k * a.nranks + a.rank;
-            const long long first = lb + c * a.chunk_wave;
-            if (first >= le)
-                break;
-            const long long last = min(first + (long long)a.chunk_wave, le);
-            for (long long i = first; i < last; i++)
-                wave_vertex<OffT, kMode>(a, a.list[i], wtab, mark, acc, probes);
+    // Job order: with an XCD class split, start with the job of this
+    // workgroup's XCD (hardware XCC_ID) so each XCD's L2 serves one class of
+    // probe lists; then help the other jobs until every queue is empty.
+    int first_job = 0;
+    if (a.xcd_jobs && a.njobs > 1)
+        first_job = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) % (unsigned)a.njobs);  // XCC_ID[3:0]
+    for (int jj = 0; jj < a.njobs; jj++) {
+        TcArgs<OffT> b = a;
+        const TcJob<OffT> &J = a.jobs[(first_job + jj) % a.njobs];
+        b.begW = J.begW;
+        b.endW = J.endW;
+        b.adjW = J.adjW;
+        b.list = J.list;
+        b.ctl = J.ctl;
+        if (b.spill_tier >= 0 && (int)blockIdx.x < b.spill_groups) {
+            int *tab = b.spill_mem + (long long)blockIdx.x * b.spill_slots;
+            int cap_lg = 0;
+            while ((2ll << cap_lg) <= b.spill_slots)
+                cap_lg++;
+            group_phase<OffT, kMode, true>(b, b.spill_tier, tab, cap_lg, mark, misc, ctot, acc,
+                                           probes);
         }
+        if (b.group_tier >= 0)
+            group_phase<OffT, kMode, false>(b, b.group_tier, lds, b.group_lg, mark, misc, ctot,
+                                            acc, probes);
+        __syncthreads();
+        if (b.wave_tier >= 0) {
+            const long long lb = b.ctl->list_begin[b.wave_tier];
+            const long long le = b.ctl->list_begin[b.wave_tier + 1];
+            int *wtab = lds + wave * b.wave_slots;
+            for (;;) {
+                unsigned int k = 0;
+                if (lane == 0)
+                    k = atomicAdd(&b.ctl->next[b.wave_tier], 1u);
+                k = __builtin_amdgcn_readfirstlane(k);
+                const long long c = (long long)k * b.nranks + b.rank;
+                const long long first = lb + c * b.chunk_wave;
+                if (first >= le)
+                    break;
+                const long long last = min(first + (long long)b.chunk_wave, le);
+                for (long long i = first; i < last; i++)
+                    wave_vertex<OffT, kMode>(b, b.list[i], wtab, mark, acc, probes);
+            }
+        }
+        __syncthreads();
     }
     acc = wave_sum(acc);
     probes = wave_sum(probes) / kWave;  // every lane carried the wave's probe total
-    if (lane == 0) {
+    if (lane == 0) {  // totals accumulate in job 0's control block
         if (acc)
-            atomicAdd(&a.ctl->count, acc);
+            atomicAdd(&a.jobs[0].ctl->count, acc);
         if (probes)
-            atomicAdd(&a.ctl->probes, probes);
+            atomicAdd(&a.jobs[0].ctl->probes, probes);
     }
 }
 
@@ -657,8 +674,8 @@ __device__ __forceinline__ int plan_tier(const PlanArgs<OffT> &p, long long u)
     if (p.tier_override)
         return (int)p.tier_override[u];
     const long long dH = (long long)(p.begH[u + 1] - p.begH[u]);
-    const long long dW = (long long)(p.begW[u + 1] - p.begW[u]);
-    if (dH == 0 || dW == 0 || (p.same_csr && dW < 2))
+    const long long dW = (long long)(p.endW[u] - p.begW[u]);
+    if (dH == 0 || dW == 0 || (p.same_csr && dH < 2))
         return kTierNone;  // contributes nothing (reference skips d+ < 2, TC.cu:440-441)
     int lg = kMinTableLg;
     while ((1ll << lg) < 2 * dH)
@@ -782,6 +799,75 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
          i += (long long)gridDim.x * blockDim.x)
         dst[i] = (int32_t)src[i];
 }
+
+// Class-ordered rows for the XCD split: row u of `out` holds the entries of
+// row u of `adj` grouped by v % C (stable inside a class); starts[(x-1)*V+u]
+// is where class x begins in row u (class 0 begins at beg[u], class C ends
+// at beg[u+1]).  One wave per row.
+template <typename OffT>
+__global__ void __launch_bounds__(256) class_layout_kernel(const OffT *__restrict__ beg,
+                                                           const int32_t *__restrict__ adj,
+                                                           int32_t *__restrict__ out, long long V,
+                                                           int C, OffT *__restrict__ starts)
+{
+    const long long waves = (long long)gridDim.x * (blockDim.x / kWave);
+    const int lane = lane_id();
+    for (long long u = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < V;
+         u += waves) {
+        const OffT s = beg[u];
+        const int d = (int)(beg[u + 1] - s);
+        int cnt[kMaxJobs];
+#pragma unroll
+        for (int c = 0; c < kMaxJobs; c++)
+            cnt[c] = 0;
+        for (int i = 0; i < d; i += kWave) {
+            const bool valid = i + lane < d;
+            const int cls = valid ? (int)((unsigned int)adj[s + (OffT)(i + lane)] % (unsigned)C) : -1;
+#pragma unroll
+            for (int c = 0; c < kMaxJobs; c++)
+                cnt[c] += (int)__popcll(__ballot(cls == c));
+        }
+        OffT off[kMaxJobs];
+        OffT run = s;
+#pragma unroll
+        for (int c = 0; c < kMaxJobs; c++) {
+            off[c] = run;
+            run += (OffT)cnt[c];
+            if (c >= 1 && c < C && lane == 0)
+                starts[(long long)(c - 1) * V + u] = off[c];
+        }
+        for (int i = 0; i < d; i += kWave) {
+            const bool valid = i + lane < d;
+            const int val = valid ? adj[s + (OffT)(i + lane)] : 0;
+            const int cls = valid ? (int)((unsigned int)val % (unsigned)C) : -1;
+#pragma unroll
+            for (int c = 0; c < kMaxJobs; c++) {
+                const unsigned long long m = __ballot(cls == c);
+                if (cls == c) {
+                    const int r = (int)__builtin_amdgcn_mbcnt_hi(
+                        (unsigned int)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0));
+                    out[off[c] + (OffT)r] = val;
+                }
+                off[c] += (OffT)__popcll(m);
+            }
+        }
+    }
+}
+
+template <typename OffT>
+hipError_t launch_class_layout(const OffT *beg, const int32_t *adj, int32_t *out, long long V,
+                               int C, OffT *starts, int blocks, hipStream_t stream)
+{
+    if (V <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(class_layout_kernel<OffT>, dim3(blocks), dim3(256), 0, stream, beg, adj, out,
+                       V, C, starts);
+    return hipGetLastError();
+}
+template hipError_t launch_class_layout<int32_t>(const int32_t *, const int32_t *, int32_t *,
+                                                 long long, int, int32_t *, int, hipStream_t);
+template hipError_t launch_class_layout<int64_t>(const int64_t *, const int32_t *, int32_t *,
+                                                 long long, int, int64_t *, int, hipStream_t);
 
 // Plain streaming copy: the known-traffic kernel that calibrates HBM byte
 // counters and gives the sequential-bandwidth reference.
