@@ -36,6 +36,7 @@ extern "C" {
 #define TRUST_FLAG_WAVE_ONLY 8u    /* no workgroup tier: every vertex that fits a wave table uses one wave */
 #define TRUST_FLAG_LINEAR_PROBING 16u /* linear-probing tables instead of two-choice cuckoo buckets */
 #define TRUST_FLAG_XCD_SPLIT 32u   /* 8 jobs by v % 8, job x first on XCD x (per-XCD L2 locality) */
+#define TRUST_FLAG_HIP_GRAPH 64u   /* batch counts: replay plan + count as one captured hipGraph */
 
 /*
  * Launch and schedule parameters.  threads / blocks / chunk / rank /
@@ -203,6 +204,39 @@ int trust_ctx_gather(trust_ctx *ctx, const trust_params *p, trust_result *r);
  * (the known-traffic kernel that calibrates HBM byte counters). */
 int trust_membench_copy(int device, int64_t bytes, int reps, double *seconds_per_copy);
 
+/* ---- batches of graphs (no counterpart in the reference) -------------- */
+/*
+ * The reference counts one graph per process (Without-graph-partition/
+ * mpi_main.cpp:72-96).  A batch is many graphs concatenated into one CSR
+ * with global ids: graph g owns vertices [vbase[g], vbase[g+1]) and every
+ * edge stays inside its graph.  One persistent count covers the whole batch
+ * and attributes each vertex's triangles to its graph.
+ *
+ * Shapes of a seeded mixed batch: graph g has n[g] vertices (log-uniform in
+ * [nmin, nmax]), mean degree d[g] (uniform in [dmin, dmax]) and kind[g] = g & 1
+ * (0: Erdos-Renyi G(n, m = n*d/2), 1: Barabasi-Albert with k = round(d/2)).
+ */
+int trust_batch_shape(int64_t ngraphs, uint64_t seed, int64_t nmin, int64_t nmax, double dmin,
+                      double dmax, int64_t *n, double *d, int32_t *kind);
+/* Generates that batch, each graph through trust_preprocess_edges, and
+ * concatenates it into *out; vbase has ngraphs + 1 entries. */
+int trust_batch_generate(int64_t ngraphs, uint64_t seed, int64_t nmin, int64_t nmax, double dmin,
+                         double dmax, trust_csr *out, int64_t *vbase);
+/* Copies a batch CSR and its vbase[ngraphs + 1] to the device (flags as for
+ * trust_ctx_load_csr; the graph is also the context's resident graph). */
+int trust_ctx_load_batch(trust_ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t vertices,
+                         int64_t edges, const int64_t *vbase, int64_t ngraphs, uint32_t flags);
+/*
+ * Counts every graph of the resident batch in one pass: counts[g] (uint64)
+ * and done_seconds[g] = time from the first workgroup's start to the end of
+ * graph g's last vertex (0 for a graph with nothing to count); both nullable.
+ * With TRUST_FLAG_HIP_GRAPH in p->flags the planning and counting launches
+ * are captured once into a hipGraph (re-captured when the parameters or
+ * buffers change) and each call replays it.
+ */
+int trust_ctx_count_batch(trust_ctx *ctx, const trust_params *p, uint64_t *counts,
+                          double *done_seconds, trust_result *r);
+
 /* ---- input contract: files, preprocessing, tiles ---------------------- */
 /* Read <folder>/begin.bin + adjacent.bin (Without-graph-partition/graph.hpp:18-67)
  * with size and content validation. */
@@ -239,8 +273,10 @@ int trust_gen_chung_lu(int64_t n, int64_t m, double gamma, double max_degree, ui
                        int32_t *src, int32_t *dst);
 /* Erdos-Renyi G(n, m): m uniform pairs. */
 int trust_gen_erdos_renyi(int64_t n, int64_t m, uint64_t seed, int32_t *src, int32_t *dst);
-/* Barabasi-Albert: n vertices, k edges per new vertex; returns k*(n-k) pairs
- * (caller allocates k*n). *m_out receives the pair count. */
+/* Barabasi-Albert (preferential attachment by sampling earlier pair ends):
+ * n vertices, k pairs per vertex, n*k pairs in all (caller allocates k*n);
+ * self-loops and repeated pairs occur and preprocessing removes them.
+ * *m_out receives the pair count. */
 int trust_gen_barabasi_albert(int64_t n, int k, uint64_t seed, int32_t *src, int32_t *dst,
                               int64_t *m_out);
 

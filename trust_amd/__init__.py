@@ -28,6 +28,7 @@ ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2
 SCHED_DEGREE, SCHED_LEARNED = 0, 1
 FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
 FLAG_XCD_SPLIT = 32
+FLAG_HIP_GRAPH = 64
 
 
 class TrustError(RuntimeError):
@@ -110,6 +111,14 @@ SIGNATURES = {
     "trust_ctx_set_cost_model": (C.c_int, [C.c_void_p, C.c_int, _PF]),
     "trust_ctx_get_cost_model": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _PF]),
     "trust_ctx_predict_costs": (C.c_int, [C.c_void_p, _PF]),
+    "trust_batch_shape": (C.c_int, [C.c_int64, C.c_uint64, C.c_int64, C.c_int64, C.c_double,
+                                    C.c_double, _P64, C.POINTER(C.c_double), _P32]),
+    "trust_batch_generate": (C.c_int, [C.c_int64, C.c_uint64, C.c_int64, C.c_int64, C.c_double,
+                                       C.c_double, C.POINTER(Csr), _P64]),
+    "trust_ctx_load_batch": (C.c_int, [C.c_void_p, _P64, _P32, C.c_int64, C.c_int64, _P64,
+                                       C.c_int64, C.c_uint32]),
+    "trust_ctx_count_batch": (C.c_int, [C.c_void_p, C.POINTER(Params), _PU64,
+                                        C.POINTER(C.c_double), C.POINTER(Result)]),
     "trust_membench_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "trust_csr_load": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "trust_csr_save": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
@@ -286,6 +295,37 @@ def rmat_csr(scale: int, edge_factor: int = 16, seed: int = 1):
     return preprocess_edges(*gen_rmat(scale, edge_factor, seed))
 
 
+# ---------------------------------------------------------------- batches
+def batch_shape(ngraphs: int, seed: int = 1, nmin: int = 1000, nmax: int = 200000,
+                dmin: float = 8.0, dmax: float = 64.0):
+    """Per-graph (n, mean degree, kind) of a seeded batch; kind 0 ER, 1 BA."""
+    n = np.zeros(ngraphs, np.int64)
+    d = np.zeros(ngraphs, np.float64)
+    k = np.zeros(ngraphs, np.int32)
+    _check(lib().trust_batch_shape(ngraphs, seed, nmin, nmax, dmin, dmax, n.ctypes.data_as(_P64),
+                                   d.ctypes.data_as(C.POINTER(C.c_double)),
+                                   k.ctypes.data_as(_P32)))
+    return n, d, k
+
+
+def batch_generate(ngraphs: int, seed: int = 1, nmin: int = 1000, nmax: int = 200000,
+                   dmin: float = 8.0, dmax: float = 64.0):
+    """A seeded ER/BA batch as one CSR with global ids: (beg, adj, vbase)."""
+    c = Csr()
+    vbase = np.zeros(ngraphs + 1, np.int64)
+    _check(lib().trust_batch_generate(ngraphs, seed, nmin, nmax, dmin, dmax, C.byref(c),
+                                      vbase.ctypes.data_as(_P64)))
+    beg, adj = _take_csr(c)
+    return beg, adj, vbase
+
+
+def batch_graph(beg, adj, vbase, g: int):
+    """Graph g of a batch as its own CSR with local ids."""
+    lo, hi = int(vbase[g]), int(vbase[g + 1])
+    b = np.asarray(beg[lo: hi + 1], np.int64)
+    return b - b[0], np.asarray(adj[b[0]: b[-1]], np.int32) - np.int32(lo)
+
+
 # ---------------------------------------------------------------- counting
 def count_csr(beg, adj, **params) -> Result:
     beg, pb = _i64(beg)
@@ -332,7 +372,7 @@ class Context:
         h = C.c_void_p()
         _check(lib().trust_ctx_create(device, C.byref(h)))
         self._h = h
-        self.V = self.E = 0
+        self.V = self.E = self.ngraphs = 0
 
     def close(self):
         if self._h:
@@ -365,6 +405,25 @@ class Context:
         r = Result()
         _check(lib().trust_ctx_count(self._h, C.byref(p), C.byref(r)))
         return r
+
+    def load_batch(self, beg, adj, vbase, flags: int = 0):
+        beg, pb = _i64(beg)
+        adj, pa = _i32(adj)
+        vbase, pv = _i64(vbase)
+        _check(lib().trust_ctx_load_batch(self._h, pb, pa, len(beg) - 1, len(adj), pv,
+                                          len(vbase) - 1, flags))
+        self.V, self.E, self.ngraphs = len(beg) - 1, len(adj), len(vbase) - 1
+
+    def count_batch(self, **params):
+        """One count of the resident batch: (counts[ng] uint64, done_seconds[ng], Result)."""
+        p = make_params(**params)
+        r = Result()
+        counts = np.zeros(self.ngraphs, np.uint64)
+        done = np.zeros(self.ngraphs, np.float64)
+        _check(lib().trust_ctx_count_batch(self._h, C.byref(p), counts.ctypes.data_as(_PU64),
+                                           done.ctypes.data_as(C.POINTER(C.c_double)),
+                                           C.byref(r)))
+        return counts, done, r
 
     def gather(self, **params) -> Result:
         """Gather-only pass (same walk, no hashing): the load-path time."""

@@ -1,8 +1,10 @@
 // Host-engine internals shared by engine.hip, partition_driver.hip,
-// learned.hip and batch.hip.
+// learned.hip and batch_driver.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <vector>
 
 #include "internal.h"
 #include "tc_kernels.h"
@@ -77,6 +79,9 @@ struct LaunchPlan {
     int spill_groups = 0;
 };
 
+// Vertices per page of the batch's graph lookup table.
+constexpr int kGraphPageShift = 12;
+
 struct Ctx {
     int device = 0;
     bool ready = false;
@@ -95,6 +100,13 @@ struct Ctx {
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     bool mlp_ready = false;
+    // batch of graphs: vbase[ngraphs + 1]; out = counts[ng], done[ng], t0
+    DevBuf batch_vbase, batch_page, batch_out;
+    int64_t batch_ngraphs = 0;
+    double wall_hz = 0;
+    // captured plan + count of the batch (TRUST_FLAG_HIP_GRAPH)
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uintptr_t> graph_key;
     ~Ctx();
 };
 
@@ -118,6 +130,25 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
 int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
                    trust_result *r);
+
+// One count over the resident graph split into its steps, so that the
+// enqueue step can also be captured into a hipGraph.
+struct ResidentRun {
+    LaunchPlan L;
+    JobSet js;
+    const JobSet *jsp = nullptr;
+    int nj = 1;
+    bool learned = false;
+};
+int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
+// Releases the captured batch graph (after a new load or at teardown).
+void drop_captured(Ctx *ctx);
+// Enqueues the fills, plan and counting launches on ctx->stream; `timed`
+// records ctx->ev[0..2] around plan and count (not under capture).
+int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRun &R, bool timed);
+// Fills *r from the control blocks copied back after the count.
+void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl *hj, double t_plan,
+                     double t_count, trust_result *r);
 
 // Learned scheduler (learned.hip): enqueues features, MLP inference, tier
 // choice and the longest-first counting sort that fill ctx->list and

@@ -221,6 +221,16 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.ctl = ctl_dev;
     a.instr_cycles = instr_cycles;
     a.instr_probes = instr_probes;
+    if (ctx->batch_ngraphs > 0) {  // read by the batch mode only
+        const int64_t ng = ctx->batch_ngraphs;
+        a.graph_vbase = (const long long *)ctx->batch_vbase.p;
+        a.graph_page = (const int *)ctx->batch_page.p;
+        a.graph_page_shift = kGraphPageShift;
+        a.ngraphs = (int)ng;
+        a.graph_counts = (unsigned long long *)ctx->batch_out.p;
+        a.graph_done = a.graph_counts + ng;
+        a.graph_t0 = a.graph_counts + 2 * ng;
+    }
 }
 
 template <typename OffT>
@@ -367,10 +377,7 @@ static int ensure_class_layout(Ctx *ctx, int C)
     return TRUST_OK;
 }
 
-// Count over the context's resident graph (or a caller-provided view set).
-int count_resident(Ctx *ctx, const trust_params &p, int mode,
-                   unsigned long long *cycles_out, unsigned long long *probes_out,
-                   trust_result *r)
+int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
 {
     int rc = check_params(p);
     if (rc)
@@ -381,13 +388,13 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
     if (g.max_deg > (int64_t)(INT32_MAX / 64))
         return fail(TRUST_ERR_OVERFLOW, "max out-degree %lld too large for one probe group",
                     (long long)g.max_deg);
-    LaunchPlan L;
-    rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, L);
+    if (mode == kModeBatch && (ctx->batch_ngraphs < 1 || !ctx->batch_vbase.p))
+        return fail(TRUST_ERR_ARG, "context holds no batch");
+    rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, R.L);
     if (rc)
         return rc;
     // XCD class split: job x walks the v with v % 8 == x, on XCD x first
-    JobSet js;
-    const JobSet *jsp = nullptr;
+    R.jsp = nullptr;
     if ((p.flags & TRUST_FLAG_XCD_SPLIT) && g.V > 0) {
         const int C = kMaxJobs;
         rc = ensure_class_layout(ctx, C);
@@ -395,58 +402,118 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
             return rc;
         const size_t ob = g.off64 ? 8 : 4;
         const char *starts = (const char *)ctx->class_starts.p;
-        js.n = C;
-        js.xcd = true;
+        R.js.n = C;
+        R.js.xcd = true;
         for (int x = 0; x < C; x++) {
-            js.begW[x] = x == 0 ? g.beg.p : starts + ob * (size_t)(x - 1) * g.V;
-            js.endW[x] = x == C - 1 ? (const void *)((const char *)g.beg.p + ob)
-                                    : starts + ob * (size_t)x * g.V;
-            js.adjW[x] = (const int32_t *)g.adj.p;
+            R.js.begW[x] = x == 0 ? g.beg.p : starts + ob * (size_t)(x - 1) * g.V;
+            R.js.endW[x] = x == C - 1 ? (const void *)((const char *)g.beg.p + ob)
+                                      : starts + ob * (size_t)x * g.V;
+            R.js.adjW[x] = (const int32_t *)g.adj.p;
         }
-        jsp = &js;
+        R.jsp = &R.js;
     }
-    const int nj = jsp ? jsp->n : 1;
-    rc = ctx->ctl.ensure(sizeof(TcCtl) * nj);
+    R.nj = R.jsp ? R.jsp->n : 1;
+    rc = ctx->ctl.ensure(sizeof(TcCtl) * R.nj);
     if (rc)
         return rc;
-    const bool instrument = mode == kModeInstr;
-    if (instrument) {
+    if (mode == kModeInstr) {
         rc = ctx->instr_cycles.ensure(sizeof(unsigned long long) * (size_t)std::max<int64_t>(g.V, 1));
         if (!rc)
             rc = ctx->instr_probes.ensure(sizeof(unsigned long long) *
                                           (size_t)std::max<int64_t>(g.V, 1));
         if (rc)
             return rc;
-        HIPCHK(hipMemsetAsync(ctx->instr_cycles.p, 0, sizeof(unsigned long long) * g.V, ctx->stream));
-        HIPCHK(hipMemsetAsync(ctx->instr_probes.p, 0, sizeof(unsigned long long) * g.V, ctx->stream));
     }
     // learned schedule when a cost model is loaded, else the degree split
     // (the learned queue is single-job: with the XCD split it is not used)
-    const bool learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready && !jsp;
+    R.learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready && !R.jsp;
+    return TRUST_OK;
+}
+
+int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRun &R, bool timed)
+{
+    DevCsr &g = ctx->g;
+    // buffers are cleared by fill kernels, which a captured graph replays
+    // correctly (hipMemsetAsync nodes are avoided)
+    if (mode == kModeInstr && g.V > 0) {
+        HIPCHK(launch_fill_u32((uint32_t *)ctx->instr_cycles.p, 0u, 2 * g.V, ctx->stream));
+        HIPCHK(launch_fill_u32((uint32_t *)ctx->instr_probes.p, 0u, 2 * g.V, ctx->stream));
+    }
+    if (mode == kModeBatch) {
+        uint32_t *out = (uint32_t *)ctx->batch_out.p;
+        HIPCHK(launch_fill_u32(out, 0u, 4 * ctx->batch_ngraphs, ctx->stream));
+        HIPCHK(launch_fill_u32(out + 4 * ctx->batch_ngraphs, 0xffffffffu, 2, ctx->stream));
+    }
     CsrViews v{g.beg.p, (const int32_t *)g.adj.p, g.beg.p, (const int32_t *)g.adj.p, g.beg.p,
                (const int32_t *)g.adj.p};
-    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (learned) {
-        rc = enqueue_learned_plan(ctx, p, L, (TcCtl *)ctx->ctl.p, ctx->stream);
+    if (timed)
+        HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    int rc;
+    if (R.learned) {
+        rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream);
         if (rc)
             return rc;
     }
+    hipEvent_t e1 = timed ? ctx->ev[1] : nullptr, e2 = timed ? ctx->ev[2] : nullptr;
     if (g.off64)
-        rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
+        rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
-                                    (unsigned long long *)ctx->instr_probes.p, learned,
-                                    ctx->stream, ctx->ev[1], ctx->ev[2], jsp);
+                                    (unsigned long long *)ctx->instr_probes.p, R.learned,
+                                    ctx->stream, e1, e2, R.jsp);
     else
-        rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, L, (TcCtl *)ctx->ctl.p, mode,
+        rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
-                                    (unsigned long long *)ctx->instr_probes.p, learned,
-                                    ctx->stream, ctx->ev[1], ctx->ev[2], jsp);
+                                    (unsigned long long *)ctx->instr_probes.p, R.learned,
+                                    ctx->stream, e1, e2, R.jsp);
+    return rc;
+}
+
+void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl *hj, double t_plan,
+                     double t_count, trust_result *r)
+{
+    if (!r)
+        return;
+    const DevCsr &g = ctx->g;
+    const LaunchPlan &L = R.L;
+    memset(r, 0, sizeof(*r));
+    r->vertices = g.V;
+    r->edges = g.E;
+    r->triangles = hj[0].count;
+    r->probes = hj[0].probes;
+    r->plan_seconds = t_plan;
+    r->count_seconds = t_count;
+    r->seconds = t_plan + t_count;
+    r->h2d_seconds = ctx->h2d_seconds;
+    r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
+    r->alg_bytes = alg_bytes(g.V, g.E, hj[0].probes, g.off64);
+    for (int j = 0; j < R.nj; j++) {  // vertex-job pairs per tier
+        r->n_spill += hj[j].list_begin[kTierSpill + 1] - hj[j].list_begin[kTierSpill];
+        r->n_hub += hj[j].list_begin[kTierHub + 1] - hj[j].list_begin[kTierBig];  // big + hub
+        r->n_wave += hj[j].list_begin[kTierWave + 1] - hj[j].list_begin[kTierWave];
+    }
+    r->threads = L.main.threads;
+    r->blocks = L.main.blocks;
+    r->hub_slots = 1 << L.main.group_lg;
+    r->wave_slots = L.main.wave_slots;
+    r->schedule_used = R.learned ? TRUST_SCHED_LEARNED : TRUST_SCHED_DEGREE;
+}
+
+// Count over the context's resident graph.
+int count_resident(Ctx *ctx, const trust_params &p, int mode,
+                   unsigned long long *cycles_out, unsigned long long *probes_out,
+                   trust_result *r)
+{
+    ResidentRun R;
+    int rc = prepare_resident(ctx, p, mode, R);
     if (rc)
         return rc;
+    rc = enqueue_resident(ctx, p, mode, R, true);
+    if (rc)
+        return rc;
+    const DevCsr &g = ctx->g;
     TcCtl hj[kMaxJobs];
-    HIPCHK(hipMemcpyAsync(hj, ctx->ctl.p, sizeof(TcCtl) * nj, hipMemcpyDeviceToHost, ctx->stream));
-    TcCtl &h = hj[0];
-    if (instrument) {
+    HIPCHK(hipMemcpyAsync(hj, ctx->ctl.p, sizeof(TcCtl) * R.nj, hipMemcpyDeviceToHost, ctx->stream));
+    if (mode == kModeInstr) {
         if (cycles_out)
             HIPCHK(hipMemcpyAsync(cycles_out, ctx->instr_cycles.p,
                                   sizeof(unsigned long long) * g.V, hipMemcpyDeviceToHost,
@@ -460,29 +527,7 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
     float t_plan = 0, t_count = 0;
     HIPCHK(hipEventElapsedTime(&t_plan, ctx->ev[0], ctx->ev[1]));
     HIPCHK(hipEventElapsedTime(&t_count, ctx->ev[1], ctx->ev[2]));
-    if (r) {
-        memset(r, 0, sizeof(*r));
-        r->vertices = g.V;
-        r->edges = g.E;
-        r->triangles = h.count;
-        r->probes = h.probes;
-        r->plan_seconds = t_plan * 1e-3;
-        r->count_seconds = t_count * 1e-3;
-        r->seconds = r->plan_seconds + r->count_seconds;
-        r->h2d_seconds = ctx->h2d_seconds;
-        r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
-        r->alg_bytes = alg_bytes(g.V, g.E, h.probes, g.off64);
-        for (int j = 0; j < nj; j++) {  // vertex-job pairs per tier
-            r->n_spill += hj[j].list_begin[kTierSpill + 1] - hj[j].list_begin[kTierSpill];
-            r->n_hub += hj[j].list_begin[kTierHub + 1] - hj[j].list_begin[kTierBig];  // big + hub
-            r->n_wave += hj[j].list_begin[kTierWave + 1] - hj[j].list_begin[kTierWave];
-        }
-        r->threads = L.main.threads;
-        r->blocks = L.main.blocks;
-        r->hub_slots = 1 << L.main.group_lg;
-        r->wave_slots = L.main.wave_slots;
-        r->schedule_used = learned ? TRUST_SCHED_LEARNED : TRUST_SCHED_DEGREE;
-    }
+    finish_resident(ctx, R, hj, t_plan * 1e-3, t_count * 1e-3, r);
     return TRUST_OK;
 }
 
@@ -516,6 +561,7 @@ Ctx::~Ctx()
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
     (void)hipStreamSynchronize(copy_stream);
+    drop_captured(this);
     g.beg.release();
     g.adj.release();
     for (auto &ev : this->ev)
@@ -533,6 +579,11 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
             return rc;
     }
     HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // state derived from the previous graph
+    drop_captured(ctx);
+    ctx->class_n = 0;
+    ctx->batch_ngraphs = 0;
     auto t0 = std::chrono::steady_clock::now();
     int rc = upload_csr(ctx, ctx->g, beg, adj, V, E, (flags & TRUST_FLAG_OFFSETS64) != 0,
                         ctx->stream);

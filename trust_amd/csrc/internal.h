@@ -42,6 +42,11 @@ void radix_sort_u64(uint64_t *keys, uint64_t *tmp, size_t n, int key_bits);
 int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr &out,
                      int64_t n_fixed = -1);
 
+// A batch CSR: vbase[0] = 0, non-decreasing, vbase[ng] = V, and every edge
+// of graph g stays inside [vbase[g], vbase[g+1]) (batch.cpp, OpenMP).
+int batch_validate(const int64_t *beg, const int32_t *adj, int64_t V, const int64_t *vbase,
+                   int64_t ngraphs);
+
 // 2D hash partition tiles, see partition.cpp.
 int partition_tiles(const int64_t *beg, const int32_t *adj, int64_t V, int n,
                     std::vector<HostCsr> &tiles);

@@ -24,6 +24,8 @@ constexpr int kUnroll = 4;           // probe windows in flight per wave
 // gather only (the same loads, every hash lookup replaced by a checksum of
 // the loaded ids: the load path alone, used as the roofline denominator).
 constexpr int kModeCount = 0, kModeInstr = 1, kModeGather = 2;
+// kModeBatch: kModeCount plus per-graph attribution (TcArgs::graph_*)
+constexpr int kModeBatch = 3;
 
 // Device control block: queue cursors, list bounds and results.  Cleared
 // by a fill kernel before a run.
@@ -81,6 +83,15 @@ struct TcArgs {
     int rank;
     int nranks;
     int force_linear;        // 1: linear-probing tables only (no cuckoo)
+    // batch of graphs (nullable): graph g owns vertices [vbase[g], vbase[g+1])
+    const long long *graph_vbase;
+    // graph_page[p] = the graph holding vertex min(p << shift, V - 1)
+    const int *graph_page;
+    int graph_page_shift;
+    int ngraphs;
+    unsigned long long *graph_counts;  // per-graph triangles
+    unsigned long long *graph_done;    // per-graph completion (wall-clock ticks)
+    unsigned long long *graph_t0;      // earliest workgroup start (wall-clock ticks)
     TcCtl *ctl;
     unsigned long long *instr_cycles;  // per vertex (instrumented kernel only)
     unsigned long long *instr_probes;

@@ -98,6 +98,60 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
     return x;
 }
 
+__device__ __forceinline__ long long uniform(long long x)
+{
+    const int lo = __builtin_amdgcn_readfirstlane((int)(x & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return ((long long)hi << 32) | (unsigned int)lo;
+}
+
+// Per-wave attribution of triangles to the graphs of a batch: the wave
+// keeps the vertex range of its current graph and per-lane running sums,
+// and adds their total (with a completion time stamp) when it moves to
+// another graph.  Called with the whole wave active and u wave-uniform.
+struct GraphAcc {
+    int g = -1;
+    long long lo = 0, hi = 0;
+    unsigned long long sum = 0;  // this lane's share
+};
+
+template <typename Args>
+__device__ __forceinline__ void graph_flush(const Args &a, GraphAcc &ga)
+{
+    const unsigned long long s = wave_sum(ga.sum);
+    if (ga.g >= 0 && __lane_id() == 0) {
+        if (s)
+            atomicAdd(a.graph_counts + ga.g, s);
+        atomicMax(a.graph_done + ga.g, (unsigned long long)wall_clock64());
+    }
+    ga.g = -1;
+    ga.sum = 0;
+}
+
+template <typename Args>
+__device__ __forceinline__ void graph_add(const Args &a, GraphAcc &ga, long long u,
+                                          unsigned int lane_hits)
+{
+    if (u < ga.lo || u >= ga.hi || ga.g < 0) {
+        graph_flush(a, ga);
+        // the page of u bounds the search: vbase[lo] <= u < vbase[hi]
+        const long long pg = u >> a.graph_page_shift;
+        int lo = a.graph_page[pg], hi = a.graph_page[pg + 1] + 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.graph_vbase[mid] <= u)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        // wave-uniform values: keep them in scalar registers
+        ga.g = __builtin_amdgcn_readfirstlane(lo);
+        ga.lo = uniform(a.graph_vbase[ga.g]);
+        ga.hi = uniform(a.graph_vbase[ga.g + 1]);
+    }
+    ga.sum += lane_hits;
+}
+
 // ---------------------------------------------------------- hash tables
 __device__ __forceinline__ int ceil_lg(int x)  // smallest lg with 2^lg >= x (x >= 1)
 {
@@ -370,7 +424,8 @@ __device__ __forceinline__ unsigned int probe_windows(const Group<OffT> &g, int 
 template <typename OffT, bool kGlobal, int kMode>
 __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *tab, int cap_lg,
                                              int *mark, int *misc, int *ctot,
-                                             unsigned long long &acc, unsigned long long &probes)
+                                             unsigned long long &acc, unsigned long long &probes,
+                                             GraphAcc &ga)
 {
     constexpr bool kInstr = kMode == kModeInstr;
     constexpr bool kTable = kMode != kModeGather;
@@ -507,6 +562,8 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
     }
     acc += hits;
     probes += wave_probes;
+    if constexpr (kMode == kModeBatch)
+        graph_add(a, ga, u, hits);
     __syncthreads();
     if constexpr (kInstr) {
         if (lane == 0 && wave_probes)
@@ -519,7 +576,8 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
 // One wavefront counts vertex u with its own LDS table slice.
 template <typename OffT, int kMode>
 __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *tab, int *mark,
-                                            unsigned long long &acc, unsigned long long &probes)
+                                            unsigned long long &acc, unsigned long long &probes,
+                                            GraphAcc &ga)
 {
     constexpr bool kInstr = kMode == kModeInstr;
     constexpr bool kTable = kMode != kModeGather;
@@ -565,6 +623,8 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     wave_sync();
     acc += hits;
     probes += wave_probes;
+    if constexpr (kMode == kModeBatch)
+        graph_add(a, ga, u, hits);
     if constexpr (kInstr) {
         if (lane == 0) {
             a.instr_probes[u] = wave_probes;
@@ -577,7 +637,8 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
 template <typename OffT, int kMode, bool kGlobal>
 __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int *tab, int cap_lg,
                                             int *mark, int *misc, int *ctot,
-                                            unsigned long long &acc, unsigned long long &probes)
+                                            unsigned long long &acc, unsigned long long &probes,
+                                            GraphAcc &ga)
 {
     const long long lb = a.ctl->list_begin[tier], le = a.ctl->list_begin[tier + 1];
     if (le <= lb)
@@ -594,7 +655,7 @@ __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int
         const long long last = min(first + (long long)a.chunk_group, le);
         for (long long i = first; i < last; i++)
             group_vertex<OffT, kGlobal, kMode>(a, a.list[i], tab, cap_lg, mark, misc, ctot, acc,
-                                                probes);
+                                                probes, ga);
     }
 }
 
@@ -610,6 +671,9 @@ __global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
     int *ctot = misc + 8;                               // kMaxChunks + 1 words
     mark[lane] = 0;
     unsigned long long acc = 0, probes = 0;
+    GraphAcc ga;
+    if (kMode == kModeBatch && threadIdx.x == 0)
+        atomicMin(a.graph_t0, (unsigned long long)wall_clock64());
 
     // Job order: with an XCD class split, start with the job of this
     // workgroup's XCD (hardware XCC_ID) so each XCD's L2 serves one class of
@@ -631,11 +695,11 @@ __global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
             while ((2ll << cap_lg) <= b.spill_slots)
                 cap_lg++;
             group_phase<OffT, kMode, true>(b, b.spill_tier, tab, cap_lg, mark, misc, ctot, acc,
-                                           probes);
+                                           probes, ga);
         }
         if (b.group_tier >= 0)
             group_phase<OffT, kMode, false>(b, b.group_tier, lds, b.group_lg, mark, misc, ctot,
-                                            acc, probes);
+                                            acc, probes, ga);
         __syncthreads();
         if (b.wave_tier >= 0) {
             const long long lb = b.ctl->list_begin[b.wave_tier];
@@ -652,11 +716,13 @@ __global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
                     break;
                 const long long last = min(first + (long long)b.chunk_wave, le);
                 for (long long i = first; i < last; i++)
-                    wave_vertex<OffT, kMode>(b, b.list[i], wtab, mark, acc, probes);
+                    wave_vertex<OffT, kMode>(b, b.list[i], wtab, mark, acc, probes, ga);
             }
         }
         __syncthreads();
     }
+    if constexpr (kMode == kModeBatch)
+        graph_flush(a, ga);
     acc = wave_sum(acc);
     probes = wave_sum(probes) / kWave;  // every lane carried the wave's probe total
     if (lane == 0) {  // totals accumulate in job 0's control block
@@ -916,6 +982,9 @@ hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t l
     else if (mode == kModeGather)
         hipLaunchKernelGGL((tc_count_kernel<OffT, kModeGather>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
+    else if (mode == kModeBatch)
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeBatch>), dim3(blocks), dim3(threads),
+                           lds_bytes, stream, a);
     else
         hipLaunchKernelGGL((tc_count_kernel<OffT, kModeCount>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
@@ -931,6 +1000,9 @@ hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, int *blocks_
     if (mode == kModeGather)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             blocks_per_cu, tc_count_kernel<OffT, kModeGather>, threads, lds_bytes);
+    if (mode == kModeBatch)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, tc_count_kernel<OffT, kModeBatch>, threads, lds_bytes);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
         blocks_per_cu, tc_count_kernel<OffT, kModeCount>, threads, lds_bytes);
 }
