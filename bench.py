@@ -1,27 +1,36 @@
-"""Benchmark: triangle TEPS on one MI355X (R-MAT scale 26, exact count).
+"""Benchmark: triangle TEPS on one MI355X, exact counts.
 
-One step = one full count of the graph resident in HBM: planning kernels
-(tier split or learned plan) + the counting launch, timed with hipEvents on
-the launch stream inside the library and with a host clock around the K
-timed steps.  Prints one JSON line.
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat]
+                  [--scale 26] [--schedule degree|learned] [--tune SECONDS]
+                  [--cpu-seconds 15] [--no-cpu]
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26]
-                  [--schedule degree|learned] [--cpu-seconds 15] [--no-cpu]
+Workloads (one JSON line each; the default is the headline):
+  rmat      R-MAT scale S (default 26), edge factor 16, (a,b,c,d) =
+            (.57,.19,.19,.05), seed 1.  One step = one full count of the
+            graph resident in HBM: planning kernels (tier split or learned
+            plan) + the counting launches, timed with hipEvents on the launch
+            stream.  Three schedules run on the same graph and must agree: the
+            plain degree-threshold split (hub threshold 100) at the default
+            launch, the same split at the reference README's launch (1024 x
+            1024, chunk 1), and the learned schedule (headline), whose cost
+            model is trained on this graph first (reported separately); its
+            feature, inference and queue kernels run inside every timed step.
+  chung-lu  Chung-Lu n = 3.8M, 16.5M pairs, gamma 2.5, max expected degree
+            800, seed 1; degree split with autotuned launch parameters.
+  batch     10,000 graphs, half Erdos-Renyi, half Barabasi-Albert, n
+            log-uniform in [1k, 200k], mean degree uniform in [8, 64], seed 1;
+            one count of the whole batch per step, replayed as a captured
+            hipGraph, learned schedule with autotuned launch parameters.
+            Every graph's count is checked against the CPU oracle.  Reports
+            graphs/s and p50/p99 per-graph completion latency.
+  tiles     R-MAT scale S hash-partitioned into n x n tiles (the reference's
+            partition layout), all n^3 tile jobs streamed through the GPU with
+            asynchronous tile copies; the total must equal the unpartitioned
+            count.  One step = one streamed count (host-to-device copies
+            included, tiles in pinned-free pageable host memory).
 
-The graph: R-MAT scale S, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05),
-seed 1, preprocessed into the reference's degree-oriented CSR (same bytes as
-the reference's Preprocess tools).  Generation, preprocessing and the
-host-to-device copy happen before the timed region.
-
-Three schedules run on the same resident graph and must agree on the count:
-  * the plain degree-threshold split (hub threshold 100) with the default
-    launch shape,
-  * the same split at the reference README's launch (1024 x 1024, chunk 1),
-  * the learned schedule (default headline): the cost model is trained on
-    this graph first (instrumented pass + GPU training, reported separately);
-    its feature, inference and queue-building kernels run inside every timed
-    step.
-`value` is the headline schedule's GTEPS = E / device seconds per step.
+Generation, preprocessing and the initial host-to-device copy happen before
+the timed region.  `value` is the whole job's throughput on one GPU.
 """
 import argparse
 import json
@@ -29,17 +38,25 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "triangle TEPS on 1 MI355X, learned vs degree-split schedule, exact count"
 HBM_PEAK_GBS = 8000.0
+METRICS = {
+    "rmat": "triangle TEPS on 1 MI355X, learned vs degree-split schedule, exact count",
+    "chung-lu": "triangle TEPS on 1 MI355X, autotuned degree split, exact count",
+    "batch": "graphs/s on 1 MI355X, 10k-graph ER/BA batch, hipGraph replay, exact per-graph counts",
+    "tiles": "triangle TEPS on 1 MI355X, 2D-partition tiles streamed from host, exact count",
+}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# ------------------------------------------------------------------ helpers
 def cpu_baseline(beg, adj, E, W, target_seconds):
     """Time the oracle's hashed count on a strided vertex sample and scale
     to the whole graph by probe share (the oracle's cost is ~linear in
@@ -52,7 +69,6 @@ def cpu_baseline(beg, adj, E, W, target_seconds):
     _, p = oracle.count_hash_sample(beg, adj, 0, stride)
     dt = max(time.perf_counter() - t0, 1e-3)
     rate = max(p, 1) / dt
-    # pick a stride whose sample takes about target_seconds
     want_probes = rate * target_seconds
     stride = int(max(1, min(4096, W / max(want_probes, 1.0))))
     t0 = time.perf_counter()
@@ -83,31 +99,85 @@ def committed_traffic(workload):
     return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=26)
-    ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--schedule", choices=["degree", "learned"], default="learned")
-    ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--blocks", type=int, default=0)
-    ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
+def run_steps(fn, warmup, steps, label, E):
+    """warmup + timed calls of fn() -> Result; counts must be identical."""
+    ref = None
+    for _ in range(warmup):
+        r = fn()
+        ref = r.triangles if ref is None else ref
+        if r.triangles != ref:
+            raise SystemExit(f"{label}: count changed between runs")
+    res = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        res.append(fn())
+    t_end = time.perf_counter()
+    counts = {r.triangles for r in res} | ({ref} if ref is not None else set())
+    if len(counts) != 1:
+        raise SystemExit(f"{label}: counts differ across steps: {counts}")
+    dev_s = sum(r.seconds for r in res) / len(res)
+    cnt_s = sum(r.count_seconds for r in res) / len(res)
+    log(f"{label}: {res[0].triangles} triangles, {dev_s * 1e3:.2f} ms/step "
+        f"(count kernels {cnt_s * 1e3:.2f} ms, plan {(dev_s - cnt_s) * 1e3:.2f} ms), "
+        f"{E / dev_s / 1e9:.3f} GTEPS")
+    return dict(r=res[0], dev_s=dev_s, cnt_s=cnt_s, wall_ms=(t_end - t_start) / steps * 1e3)
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X only: this build runs the count on a single GPU "
-              "(multi-GPU sharding is a design note, not implemented). Use --gpus 1.",
-              file=sys.stderr)
-        sys.exit(2)
 
-    import trust_amd as T
+def roofline(T, ctx, r0, cnt_s, kw, workload):
+    """Roofline object for the counting launches: algorithmic bytes / time,
+    against HBM peak, plus the measured load-path references (the same walk
+    with hashing replaced by a checksum, and a streaming copy)."""
+    achieved = r0.alg_bytes / cnt_s / 1e9
+    gkw = dict(kw, schedule=T.SCHED_DEGREE)
+    ctx.gather(**gkw)
+    gs = [ctx.gather(**gkw).count_seconds for _ in range(3)]
+    gather_gbs = r0.alg_bytes / (sum(gs) / len(gs)) / 1e9
+    copy_bytes = 2 << 30
+    copy_s = T.membench_copy(copy_bytes, reps=5)
+    copy_gbs = 2 * copy_bytes / copy_s / 1e9
+    log(f"count kernels {cnt_s * 1e3:.2f} ms ({achieved:.0f} GB/s alg); gather-only "
+        f"{sum(gs) / len(gs) * 1e3:.2f} ms ({gather_gbs:.0f} GB/s); copy {copy_gbs:.0f} GB/s")
+    return {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": committed_traffic(workload),
+        "alg_bytes_per_launch": r0.alg_bytes,
+        "gather_only_GBs": gather_gbs,
+        "frac_of_gather_only": achieved / gather_gbs,
+        "copy_GBs": copy_gbs,
+    }
 
-    log(f"host: {os.cpu_count()} cpus, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}")
+
+def autotune(ctx, seconds, base):
+    if seconds <= 0:
+        return {}, None
+    tuned, sec, n = ctx.autotune(reps=2, max_seconds=seconds, **base)
+    log(f"autotune ({n} candidates, {seconds:.0f}s budget): {tuned} -> {sec * 1e3:.2f} ms/count")
+    return tuned, dict(params=tuned, seconds_per_count=sec, candidates=n, budget_s=seconds)
+
+
+def line_head(args, workload, value, unit, ms, dtype="int32", higher=True):
+    return {
+        "metric": METRICS[workload],
+        "value": value,
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": higher,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+    }
+
+
+# ---------------------------------------------------------------- workloads
+def bench_rmat(args, T):
     t0 = time.perf_counter()
     src, dst = T.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
     t1 = time.perf_counter()
@@ -121,114 +191,254 @@ def main():
     ctx = T.Context(0)
     h2d = ctx.load(beg, adj)
     log(f"h2d {h2d:.2f}s ({(beg.nbytes + adj.nbytes) / h2d / 1e9:.1f} GB/s incl. pageable staging)")
-
     base_kw = dict(threads=args.threads, blocks=args.blocks, chunk=args.chunk)
 
-    def run(kw, warmup, steps, label):
-        """warmup + timed steps of one schedule; counts must be identical."""
-        ref = None
-        for i in range(warmup):
-            r = ctx.count(**kw)
-            ref = r.triangles if ref is None else ref
-            if r.triangles != ref:
-                raise SystemExit(f"{label}: count changed between runs")
-        res = []
-        t_start = time.perf_counter()
-        for _ in range(steps):
-            res.append(ctx.count(**kw))
-        t_end = time.perf_counter()
-        counts = {r.triangles for r in res} | ({ref} if ref is not None else set())
-        if len(counts) != 1:
-            raise SystemExit(f"{label}: counts differ across steps: {counts}")
-        dev_s = sum(r.seconds for r in res) / len(res)
-        cnt_s = sum(r.count_seconds for r in res) / len(res)
-        log(f"{label}: {res[0].triangles} triangles, {dev_s * 1e3:.1f} ms/step "
-            f"(count kernels {cnt_s * 1e3:.1f} ms, plan {(dev_s - cnt_s) * 1e3:.2f} ms), "
-            f"{E / dev_s / 1e9:.3f} GTEPS")
-        return dict(r=res[0], dev_s=dev_s, cnt_s=cnt_s,
-                    wall_ms=(t_end - t_start) / steps * 1e3)
-
-    # the plain degree-threshold split (hub threshold 100), autotuned launch
-    deg = run(dict(base_kw, schedule=T.SCHED_DEGREE), args.warmup, args.steps, "degree split")
-    # the same split at the reference README's launch (1024 threads x 1024 blocks, chunk 1)
-    ref_launch = run(dict(schedule=T.SCHED_DEGREE, threads=1024, blocks=1024, chunk=1), 1,
-                     max(1, args.steps // 2), "degree split, reference launch")
-    # learned schedule: train the cost model on this graph (timed separately)
-    train = None
+    deg = run_steps(lambda: ctx.count(**base_kw, schedule=T.SCHED_DEGREE), args.warmup,
+                    args.steps, "degree split", E)
+    ref_kw = dict(schedule=T.SCHED_DEGREE, threads=1024, blocks=1024, chunk=1)
+    ref_launch = run_steps(lambda: ctx.count(**ref_kw), 1, max(1, args.steps // 2),
+                           "degree split, reference launch", E)
+    train = tune = None
     if args.schedule == "learned":
         mse, train_s = ctx.train_cost_model(epochs=15, max_samples=200000, seed=1, **base_kw)
         log(f"cost model trained: mse {mse:.5f} (log-cycle units/32), {train_s:.2f}s device time")
         train = dict(mse=mse, seconds=train_s)
-        head = run(dict(base_kw, schedule=T.SCHED_LEARNED), args.warmup, args.steps, "learned")
+        head_kw = dict(base_kw, schedule=T.SCHED_LEARNED)
+        tuned, tune = autotune(ctx, args.tune, head_kw)
+        head_kw.update(tuned)
+        head = run_steps(lambda: ctx.count(**head_kw), args.warmup, args.steps, "learned", E)
         if head["r"].schedule_used != T.SCHED_LEARNED:
             raise SystemExit("learned schedule did not run")
     else:
-        head = deg
+        head_kw = dict(base_kw, schedule=T.SCHED_DEGREE)
+        tuned, tune = autotune(ctx, args.tune, head_kw)
+        head_kw.update(tuned)
+        head = run_steps(lambda: ctx.count(**head_kw), args.warmup, args.steps, "degree (head)", E)
     if head["r"].triangles != deg["r"].triangles or ref_launch["r"].triangles != deg["r"].triangles:
         raise SystemExit("schedules disagree on the count")
     r0, dev_s, cnt_s = head["r"], head["dev_s"], head["cnt_s"]
     W = r0.probes
-    achieved = r0.alg_bytes / cnt_s / 1e9
-    # Load-path reference on the same graph: the same plan and walk with the
-    # hash lookups replaced by a checksum (measured random-segment bandwidth),
-    # and a plain streaming copy (measured sequential bandwidth).
-    gkw = dict(base_kw, schedule=T.SCHED_DEGREE)
-    ctx.gather(**gkw)
-    gs = [ctx.gather(**gkw).count_seconds for _ in range(3)]
-    gather_gbs = r0.alg_bytes / (sum(gs) / len(gs)) / 1e9
-    copy_bytes = 2 << 30
-    copy_s = T.membench_copy(copy_bytes, reps=5)
-    copy_gbs = 2 * copy_bytes / copy_s / 1e9
-    log(f"count kernel {cnt_s * 1e3:.1f} ms ({achieved:.0f} GB/s alg); gather-only "
-        f"{sum(gs) / len(gs) * 1e3:.1f} ms ({gather_gbs:.0f} GB/s); copy {copy_gbs:.0f} GB/s")
     workload = f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}"
-    traffic = committed_traffic(workload)
-    line = {
-        "metric": METRIC,
-        "value": E / dev_s / 1e9,
-        "unit": "GTEPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dev_s * 1e3,
-        "ms_per_step_wall": head["wall_ms"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {
-            "workload": workload,
-            "generator": "R-MAT a=0.57 b=0.19 c=0.19 d=0.05, reference preprocess contract",
-            "V": V, "E": E, "W_probes": W, "triangles": r0.triangles,
-            "schedule": args.schedule, "threads": r0.threads, "blocks": r0.blocks,
-            "tiers_spill_hub_wave": [r0.n_spill, r0.n_hub, r0.n_wave],
-            "parallelism": "single-gpu",
-            "count_kernel_ms": cnt_s * 1e3, "plan_ms": (dev_s - cnt_s) * 1e3,
-            "degree_split_gteps": E / deg["dev_s"] / 1e9,
-            "degree_split_reference_launch_gteps": E / ref_launch["dev_s"] / 1e9,
-            "speedup_vs_degree_split": deg["dev_s"] / dev_s,
-            "speedup_vs_degree_split_reference_launch": ref_launch["dev_s"] / dev_s,
-            "cost_model_training": train,
-            "h2d_s": h2d, "preprocess_s": t2 - t1,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "alg_bytes_per_launch": r0.alg_bytes,
-            "gather_only_GBs": gather_gbs,
-            "frac_of_gather_only": achieved / gather_gbs,
-            "copy_GBs": copy_gbs,
-        },
+    line = line_head(args, "rmat", E / dev_s / 1e9, "GTEPS", dev_s * 1e3)
+    line["ms_per_step_wall"] = head["wall_ms"]
+    line["config"] = {
+        "workload": workload,
+        "generator": "R-MAT a=0.57 b=0.19 c=0.19 d=0.05, reference preprocess contract",
+        "V": V, "E": E, "W_probes": W, "triangles": r0.triangles,
+        "schedule": args.schedule, "threads": r0.threads, "blocks": r0.blocks,
+        "tiers_spill_hub_wave": [r0.n_spill, r0.n_hub, r0.n_wave],
+        "parallelism": "single-gpu",
+        "count_kernel_ms": cnt_s * 1e3, "plan_ms": (dev_s - cnt_s) * 1e3,
+        "degree_split_gteps": E / deg["dev_s"] / 1e9,
+        "degree_split_reference_launch_gteps": E / ref_launch["dev_s"] / 1e9,
+        "speedup_vs_degree_split": deg["dev_s"] / dev_s,
+        "speedup_vs_degree_split_reference_launch": ref_launch["dev_s"] / dev_s,
+        "cost_model_training": train,
+        "autotune": tune,
+        "h2d_s": h2d, "preprocess_s": t2 - t1,
     }
+    line["roofline"] = roofline(T, ctx, r0, cnt_s, base_kw, workload)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(beg, adj, E, W, args.cpu_seconds)
     ctx.close()
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def bench_chung_lu(args, T):
+    n, m = 3_800_000, 16_500_000
+    t0 = time.perf_counter()
+    beg, adj = T.preprocess_edges(*T.gen_chung_lu(n, m, 2.5, 800.0, seed=args.seed))
+    V, E = len(beg) - 1, len(adj)
+    log(f"chung-lu: V={V} E={E} max d+ {np.diff(beg).max()} in {time.perf_counter() - t0:.1f}s")
+    ctx = T.Context(0)
+    ctx.load(beg, adj)
+    base_kw = dict(schedule=T.SCHED_DEGREE)
+    deg = run_steps(lambda: ctx.count(**base_kw), args.warmup, args.steps, "degree split", E)
+    ref_launch = run_steps(lambda: ctx.count(schedule=T.SCHED_DEGREE, threads=1024, blocks=1024,
+                                             chunk=1), 1, args.steps, "reference launch", E)
+    tuned, tune = autotune(ctx, args.tune if args.tune > 0 else 30.0, base_kw)
+    head_kw = dict(base_kw, **tuned)
+    head = run_steps(lambda: ctx.count(**head_kw), args.warmup, args.steps, "autotuned", E)
+    if len({deg["r"].triangles, ref_launch["r"].triangles, head["r"].triangles}) != 1:
+        raise SystemExit("launches disagree on the count")
+    r0, dev_s, cnt_s = head["r"], head["dev_s"], head["cnt_s"]
+    workload = f"chunglu3.8M_m16.5M_g2.5_dmax800_seed{args.seed}"
+    line = line_head(args, "chung-lu", E / dev_s / 1e9, "GTEPS", dev_s * 1e3)
+    line["config"] = {
+        "workload": workload, "V": V, "E": E, "W_probes": r0.probes, "triangles": r0.triangles,
+        "parallelism": "single-gpu", "count_kernel_ms": cnt_s * 1e3,
+        "plan_ms": (dev_s - cnt_s) * 1e3,
+        "default_launch_gteps": E / deg["dev_s"] / 1e9,
+        "reference_launch_gteps": E / ref_launch["dev_s"] / 1e9,
+        "speedup_vs_default_launch": deg["dev_s"] / dev_s,
+        "speedup_vs_reference_launch": ref_launch["dev_s"] / dev_s,
+        "autotune": tune,
+    }
+    line["roofline"] = roofline(T, ctx, r0, cnt_s, head_kw, workload)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(beg, adj, E, r0.probes, args.cpu_seconds)
+    ctx.close()
+    return line
+
+
+def bench_batch(args, T):
+    import oracle
+
+    ng = args.graphs
+    t0 = time.perf_counter()
+    beg, adj, vb = T.batch_generate(ng, seed=args.seed, nmin=args.nmin, nmax=args.nmax,
+                                    dmin=8.0, dmax=64.0)
+    V, E = len(beg) - 1, len(adj)
+    log(f"batch: {ng} graphs V={V} E={E} generated in {time.perf_counter() - t0:.1f}s")
+    ctx = T.Context(0)
+    t1 = time.perf_counter()
+    ctx.load_batch(beg, adj, vb)
+    log(f"loaded in {time.perf_counter() - t1:.1f}s")
+    mse, train_s = ctx.train_cost_model(epochs=15, max_samples=200000, seed=1)
+    log(f"cost model trained: mse {mse:.5f}, {train_s:.2f}s device time")
+    kw = dict(schedule=T.SCHED_LEARNED)
+    tuned, tune = autotune(ctx, args.tune if args.tune > 0 else 60.0, kw)
+    kw.update(tuned)
+    counts, done, r = ctx.count_batch(**kw)
+    if r.schedule_used != T.SCHED_LEARNED:
+        raise SystemExit("learned schedule did not run")
+    log(f"eager batch count: {r.triangles} triangles in {r.seconds * 1e3:.1f} ms")
+    # every graph against the oracle, in slices so progress shows
+    t2 = time.perf_counter()
+    want = np.zeros(ng, np.uint64)
+    step = max(1, ng // 10)
+    for g0 in range(0, ng, step):
+        g1 = min(ng, g0 + step)
+        lo, hi = int(vb[g0]), int(vb[g1])
+        b = beg[lo: hi + 1] - beg[lo]
+        want[g0:g1] = oracle.count_hash_batch(b, adj[beg[lo]: beg[hi]] - np.int32(lo),
+                                              vb[g0: g1 + 1] - lo)
+        log(f"oracle graphs [{g0}, {g1}) done, {time.perf_counter() - t2:.1f}s")
+    oracle_s = time.perf_counter() - t2
+    bad = np.nonzero(counts != want)[0]
+    if len(bad):
+        raise SystemExit(f"per-graph counts differ from the oracle at graphs {bad[:10].tolist()}")
+    log(f"all {ng} per-graph counts match the oracle ({oracle_s:.1f}s on the host)")
+    gkw = dict(kw, flags=T.FLAG_HIP_GRAPH)
+    for _ in range(args.warmup):
+        ctx.count_batch(**gkw)
+    secs, dones = [], []
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        c, d, rr = ctx.count_batch(**gkw)
+        if not np.array_equal(c, want):
+            raise SystemExit("a hipGraph replay changed a per-graph count")
+        secs.append(rr.seconds)
+        dones.append(d)
+    wall = (time.perf_counter() - t3) / args.steps
+    dev_s = float(np.mean(secs))
+    lat = np.concatenate(dones)
+    p50, p99 = np.percentile(lat, [50, 99])
+    log(f"hipGraph replay: {dev_s * 1e3:.1f} ms/batch, {ng / dev_s:.0f} graphs/s, "
+        f"{E / dev_s / 1e9:.3f} GTEPS, latency p50 {p50 * 1e3:.2f} ms p99 {p99 * 1e3:.2f} ms")
+    eager = run_steps(lambda: ctx.count_batch(**kw)[2], 0, max(1, args.steps // 2),
+                      "eager batch", E)
+    workload = f"batch{ng}_er_ba_n{args.nmin}-{args.nmax}_d8-64_seed{args.seed}"
+    line = line_head(args, "batch", ng / dev_s, "graphs/s", dev_s * 1e3)
+    line["ms_per_step_wall"] = wall * 1e3
+    line["config"] = {
+        "workload": workload, "graphs": ng, "V": V, "E": E, "W_probes": r.probes,
+        "triangles": r.triangles, "parallelism": "single-gpu", "schedule": "learned",
+        "launch": "hipGraph replay (plan + count captured once)",
+        "gteps": E / dev_s / 1e9,
+        "latency_p50_ms": p50 * 1e3, "latency_p99_ms": p99 * 1e3,
+        "latency_definition": "per graph: first workgroup start to the end of the graph's "
+                              "last vertex, device wall clock, pooled over the timed replays",
+        "eager_ms_per_batch": eager["dev_s"] * 1e3,
+        "per_graph_parity": f"all {ng} graphs equal the oracle",
+        "cost_model_training": dict(mse=mse, seconds=train_s),
+        "autotune": tune,
+    }
+    line["roofline"] = roofline(T, ctx, r, r.count_seconds, kw, workload)
+    if not args.no_cpu:
+        line["cpu_baseline"] = {
+            "value": ng / oracle_s, "unit": "graphs/s", "cores": oracle.max_threads(),
+            "kind": "port",
+            "sample": f"oracle hashed count of every one of the {ng} graphs (parallel over "
+                      f"graphs) in {oracle_s:.1f}s",
+        }
+    ctx.close()
+    return line
+
+
+def bench_tiles(args, T):
+    import oracle  # noqa: F401  (checker only: the totals are compared below)
+
+    n = args.parts
+    beg, adj = T.rmat_csr(args.scale, args.edge_factor, seed=args.seed)
+    V, E = len(beg) - 1, len(adj)
+    t0 = time.perf_counter()
+    tiles = T.partition_csr(beg, adj, n)
+    log(f"rmat{args.scale}: V={V} E={E}; {n}x{n} tiles in {time.perf_counter() - t0:.1f}s")
+    ctx = T.Context(0)
+    ctx.load(beg, adj)
+    whole = run_steps(lambda: ctx.count(), 1, 2, "unpartitioned", E)
+    part = run_steps(lambda: T.count_tiles(tiles, n), 1, args.steps, f"{n}x{n} tiles", E)
+    if part["r"].triangles != whole["r"].triangles:
+        raise SystemExit(f"partitioned {part['r'].triangles} != unpartitioned "
+                         f"{whole['r'].triangles}")
+    dev_s = part["dev_s"]
+    workload = f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}_tiles{n}x{n}"
+    tile_bytes = sum(b.nbytes + a.nbytes for b, a in tiles)
+    line = line_head(args, "tiles", E / dev_s / 1e9, "GTEPS", dev_s * 1e3)
+    line["config"] = {
+        "workload": workload, "V": V, "E": E, "triangles": part["r"].triangles,
+        "jobs": n ** 3, "tile_bytes": tile_bytes, "parallelism": "single-gpu",
+        "unpartitioned_gteps": E / whole["dev_s"] / 1e9,
+        "timing": "streamed count incl. asynchronous host-to-device tile copies",
+    }
+    r0 = part["r"]
+    achieved = r0.alg_bytes / dev_s / 1e9
+    line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": committed_traffic(workload),
+                        "alg_bytes_per_launch": r0.alg_bytes}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(beg, adj, E, whole["r"].probes, args.cpu_seconds)
+    ctx.close()
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=sorted(METRICS), default="rmat")
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--schedule", choices=["degree", "learned"], default="learned")
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--blocks", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--tune", type=float, default=0.0,
+                    help="autotune budget in seconds (0: rmat off; chung-lu 30; batch 60)")
+    ap.add_argument("--graphs", type=int, default=10000)
+    ap.add_argument("--nmin", type=int, default=1000)
+    ap.add_argument("--nmax", type=int, default=200000)
+    ap.add_argument("--parts", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X only: this build runs the count on a single GPU "
+              "(multi-GPU sharding is a design note, not implemented). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import trust_amd as T
+
+    log(f"host: {os.cpu_count()} cpus, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}")
+    fn = {"rmat": bench_rmat, "chung-lu": bench_chung_lu, "batch": bench_batch,
+          "tiles": bench_tiles}[args.workload]
+    print(json.dumps(fn(args, T)), flush=True)
 
 
 if __name__ == "__main__":

@@ -204,6 +204,21 @@ int trust_ctx_gather(trust_ctx *ctx, const trust_params *p, trust_result *r);
  * (the known-traffic kernel that calibrates HBM byte counters). */
 int trust_membench_copy(int device, int64_t bytes, int reps, double *seconds_per_copy);
 
+/* ---- launch autotuner (no counterpart in the reference) --------------- */
+/*
+ * Replaces choosing argv[3..5] (threads, blocks, chunk) of
+ * Without-graph-partition/mpi_main.cpp:78-81 by hand and the compile-time
+ * USE_CTA threshold (TC.cu:14): times counts of the resident graph with
+ * `base` as the starting point, by coordinate descent over threads
+ * {128..1024}, wave_slots {128..2048}, hub_threshold {16..2000} and chunk
+ * {auto, 1..64}, keeping a change only when it is more than 1% faster.  Each
+ * candidate is the minimum of `reps` device times; every count must equal
+ * the first (else TRUST_ERR_HIP).  max_seconds > 0 bounds the search.
+ * *best receives the fastest parameters (other fields as in base).
+ */
+int trust_ctx_autotune(trust_ctx *ctx, const trust_params *base, int reps, double max_seconds,
+                       trust_params *best, double *best_seconds, int *tried);
+
 /* ---- batches of graphs (no counterpart in the reference) -------------- */
 /*
  * The reference counts one graph per process (Without-graph-partition/

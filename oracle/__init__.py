@@ -55,6 +55,9 @@ def lib():
         h.oracle_count_hash_sample.restype = C.c_uint64
         h.oracle_count_hash_sample.argtypes = [_P64, _P32, C.c_int64, C.c_int64, C.c_int64,
                                                C.c_int, C.POINTER(C.c_uint64)]
+        h.oracle_count_hash_batch.restype = None
+        h.oracle_count_hash_batch.argtypes = [_P64, _P32, C.c_int64, _P64, C.c_int64, C.c_int,
+                                              C.POINTER(C.c_uint64)]
         h.oracle_max_threads.restype = C.c_int
         _lib = h
     return _lib
@@ -76,6 +79,17 @@ def count_hash(beg, adj, threads: int = 0, return_max_bucket: bool = False):
     mb = C.c_int64(0)
     c = lib().oracle_count_hash(pb, pa, len(beg) - 1, threads, C.byref(mb))
     return (int(c), mb.value) if return_max_bucket else int(c)
+
+
+def count_hash_batch(beg, adj, vbase, threads: int = 0):
+    """Per-graph hashed counts of a batch (uint64 array, one per graph)."""
+    beg, pb = _i64(beg)
+    adj, pa = _i32(adj)
+    vbase, pv = _i64(vbase)
+    out = np.zeros(len(vbase) - 1, np.uint64)
+    lib().oracle_count_hash_batch(pb, pa, len(beg) - 1, pv, len(vbase) - 1, threads,
+                                  out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
 
 
 def count_merge(beg, adj, threads: int = 0) -> int:

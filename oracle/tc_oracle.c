@@ -38,6 +38,10 @@
  *       (Without-graph-partition/TC.cu:354,534).
  *   oracle_features     the learned scheduler's per-vertex features
  *       d+(u), sum_{v in N+(u)} d+(v), max_{v in N+(u)} d+(v).
+ *   oracle_count_hash_batch  oracle_count_hash per graph of a batch (graph
+ *       g owns vertices [vbase[g], vbase[g+1])), parallel over graphs: the
+ *       reference counts one graph per run (Without-graph-partition/
+ *       mpi_main.cpp:72-96), so the per-graph count is that run's count.
  *
  * Pinning (tests/test_oracle.py): closed forms (K_n, K_{a,b}, C_n, wheels,
  * friendship graphs, complete tripartite), brute force on small random
@@ -331,6 +335,49 @@ void oracle_features(const int64_t *beg, const int32_t *adj, int64_t V, double *
         out[3 * u + 0] = (double)d;
         out[3 * u + 1] = s;
         out[3 * u + 2] = mx;
+    }
+}
+
+/*
+ * Per-graph hashed counts of a batch: out[g] = oracle_count_hash of graph g.
+ * Graphs run in parallel, the vertices of one graph in sequence.
+ */
+void oracle_count_hash_batch(const int64_t *beg, const int32_t *adj, int64_t V,
+                             const int64_t *vbase, int64_t ngraphs, int nthreads, uint64_t *out)
+{
+    int64_t maxd = 0;
+    for (int64_t u = 0; u < V; u++) {
+        int64_t d = beg[u + 1] - beg[u];
+        if (d > maxd)
+            maxd = d;
+    }
+    set_threads(nthreads);
+#pragma omp parallel
+    {
+        bucket_table t;
+        t.start = (int32_t *)malloc(sizeof(int32_t) * (ORACLE_CTA_BUCKETS + 1));
+        t.fill = (int32_t *)malloc(sizeof(int32_t) * ORACLE_CTA_BUCKETS);
+        t.items = (int32_t *)malloc(sizeof(int32_t) * (size_t)(maxd + 1));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t g = 0; g < ngraphs; g++) {
+            uint64_t c = 0;
+            for (int64_t u = vbase[g]; u < vbase[g + 1]; u++) {
+                int64_t d = beg[u + 1] - beg[u];
+                if (d < ORACLE_MIN_DEGREE)
+                    continue;
+                int nb = d > ORACLE_CTA_THRESHOLD ? ORACLE_CTA_BUCKETS : ORACLE_WARP_BUCKETS;
+                bt_build(&t, adj + beg[u], d, nb);
+                for (int64_t e = beg[u]; e < beg[u + 1]; e++) {
+                    int32_t v = adj[e];
+                    for (int64_t f = beg[v]; f < beg[v + 1]; f++)
+                        c += (uint64_t)bt_contains(&t, adj[f], nb);
+                }
+            }
+            out[g] = c;
+        }
+        free(t.start);
+        free(t.fill);
+        free(t.items);
     }
 }
 

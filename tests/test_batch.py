@@ -60,6 +60,22 @@ def test_per_graph_counts_sum_to_whole():
     beg, adj, vb = T.batch_generate(40, seed=4, nmin=30, nmax=1500, dmin=4, dmax=40)
     per = [oracle.count_hash(*T.batch_graph(beg, adj, vb, g)) for g in range(40)]
     assert sum(per) == oracle.count_hash(beg, adj) == oracle.count_merge(beg, adj)
+    # the batch counter of the oracle: graph by graph, each graph on its own
+    assert oracle.count_hash_batch(beg, adj, vb).tolist() == per
+    assert [oracle.count_merge(*T.batch_graph(beg, adj, vb, g)) for g in range(40)] == per
+
+
+def test_oracle_batch_counter_closed_forms():
+    names = ["K10", "W30", "K150", "C9", "F25", "K5_6_7", "K40_60"]
+    graphs = [closed_form_csr(n) for n in names]
+    parts = []
+    for b, a, _ in graphs:
+        parts += [(b, a), (np.zeros(1, np.int64), np.zeros(0, np.int32))]
+    beg, adj, vb = concat_csr(parts)
+    want = []
+    for _, _, w in graphs:
+        want += [w, 0]
+    assert oracle.count_hash_batch(beg, adj, vb).tolist() == want
 
 
 def test_concat_closed_forms():

@@ -29,6 +29,8 @@ SCHED_DEGREE, SCHED_LEARNED = 0, 1
 FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
 FLAG_XCD_SPLIT = 32
 FLAG_HIP_GRAPH = 64
+# launch parameters the autotuner searches
+TUNED_FIELDS = ("threads", "wave_slots", "hub_threshold", "chunk")
 
 
 class TrustError(RuntimeError):
@@ -111,6 +113,9 @@ SIGNATURES = {
     "trust_ctx_set_cost_model": (C.c_int, [C.c_void_p, C.c_int, _PF]),
     "trust_ctx_get_cost_model": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _PF]),
     "trust_ctx_predict_costs": (C.c_int, [C.c_void_p, _PF]),
+    "trust_ctx_autotune": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, C.c_double,
+                                     C.POINTER(Params), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int)]),
     "trust_batch_shape": (C.c_int, [C.c_int64, C.c_uint64, C.c_int64, C.c_int64, C.c_double,
                                     C.c_double, _P64, C.POINTER(C.c_double), _P32]),
     "trust_batch_generate": (C.c_int, [C.c_int64, C.c_uint64, C.c_int64, C.c_int64, C.c_double,
@@ -405,6 +410,17 @@ class Context:
         r = Result()
         _check(lib().trust_ctx_count(self._h, C.byref(p), C.byref(r)))
         return r
+
+    def autotune(self, reps: int = 2, max_seconds: float = 0.0, **base):
+        """Fastest launch parameters for the resident graph: (params dict,
+        seconds per count, candidates timed)."""
+        p = make_params(**base)
+        best = Params()
+        sec, n = C.c_double(0), C.c_int(0)
+        _check(lib().trust_ctx_autotune(self._h, C.byref(p), reps, max_seconds, C.byref(best),
+                                        C.byref(sec), C.byref(n)))
+        tuned = {k: getattr(best, k) for k in TUNED_FIELDS}
+        return tuned, sec.value, n.value
 
     def load_batch(self, beg, adj, vbase, flags: int = 0):
         beg, pb = _i64(beg)

@@ -115,8 +115,10 @@ int upload_csr(Ctx *ctx, DevCsr &g, const int64_t *beg, const int32_t *adj, int6
                bool force64, hipStream_t s);
 int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,
                  uint32_t flags);
+// Launch shapes of one count; `mode` selects the kernel whose occupancy
+// sizes the grid.
 int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_t V, bool off64,
-                   LaunchPlan &L);
+                   LaunchPlan &L, int mode = kModeCount);
 int check_params(const trust_params &p);
 double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64);
 

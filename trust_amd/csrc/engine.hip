@@ -99,11 +99,11 @@ static long long fair_table_words(long long lds_bytes, int threads)
     return share > 0 ? 1ll << floor_lg(share) : 0;
 }
 
-static int shape_occupancy(Ctx *ctx, LaunchShape &s, bool off64, int blocks_req)
+static int shape_occupancy(Ctx *ctx, LaunchShape &s, bool off64, int blocks_req, int mode)
 {
     int per_cu = 0;
-    hipError_t e = off64 ? count_occupancy<int64_t>(s.threads, s.lds_bytes, false, &per_cu)
-                         : count_occupancy<int32_t>(s.threads, s.lds_bytes, false, &per_cu);
+    hipError_t e = off64 ? count_occupancy<int64_t>(s.threads, s.lds_bytes, mode, &per_cu)
+                         : count_occupancy<int32_t>(s.threads, s.lds_bytes, mode, &per_cu);
     if (e != hipSuccess)
         return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (per_cu < 1)
@@ -114,7 +114,7 @@ static int shape_occupancy(Ctx *ctx, LaunchShape &s, bool off64, int blocks_req)
 }
 
 int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_t V,
-                   bool off64, LaunchPlan &L)
+                   bool off64, LaunchPlan &L, int mode)
 {
     L = LaunchPlan();
     LaunchShape &m = L.main;
@@ -142,7 +142,7 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
     m.table_words = (int)std::max(wave_words, main_words);
     m.group_lg = floor_lg(m.table_words);
     m.lds_bytes = (size_t)(m.table_words + lds_extra_words(m.threads)) * 4;
-    int rc = shape_occupancy(ctx, m, off64, p.blocks);
+    int rc = shape_occupancy(ctx, m, off64, p.blocks, mode);
     if (rc)
         return rc;
     if (p.chunk > 0) {
@@ -177,7 +177,7 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
             b.group_lg = 6;
             b.lds_bytes = (size_t)(64 + lds_extra_words(64)) * 4;
         }
-        rc = shape_occupancy(ctx, b, off64, 0);
+        rc = shape_occupancy(ctx, b, off64, 0, mode);
         if (rc)
             return rc;
         if (need_lg > b.group_lg || force_spill) {
@@ -390,7 +390,7 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
                     (long long)g.max_deg);
     if (mode == kModeBatch && (ctx->batch_ngraphs < 1 || !ctx->batch_vbase.p))
         return fail(TRUST_ERR_ARG, "context holds no batch");
-    rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, R.L);
+    rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, R.L, mode);
     if (rc)
         return rc;
     // XCD class split: job x walks the v with v % 8 == x, on XCD x first
