@@ -35,8 +35,8 @@ extern "C" {
 #define TRUST_FLAG_NO_VALIDATE 4u  /* skip the host-side CSR validation */
 #define TRUST_FLAG_WAVE_ONLY 8u    /* no workgroup tier: every vertex that fits a wave table uses one wave */
 #define TRUST_FLAG_LINEAR_PROBING 16u /* linear-probing tables instead of two-choice cuckoo buckets */
-#define TRUST_FLAG_XCD_SPLIT 32u   /* 8 jobs by v % 8, job x first on XCD x (per-XCD L2 locality) */
 #define TRUST_FLAG_HIP_GRAPH 64u   /* batch counts: replay plan + count as one captured hipGraph */
+/* Any other bit in trust_params.flags is rejected with TRUST_ERR_ARG. */
 
 /*
  * Launch and schedule parameters.  threads / blocks / chunk / rank /

@@ -38,7 +38,7 @@ def bctx(gpu, batch):
     {}, dict(threads=64), dict(threads=1024), dict(chunk=1), dict(chunk=7),
     dict(flags=T.FLAG_FORCE_SPILL), dict(flags=T.FLAG_WAVE_ONLY), dict(hub_threshold=0),
     dict(flags=T.FLAG_LINEAR_PROBING), dict(flags=T.FLAG_OFFSETS64),
-    dict(flags=T.FLAG_XCD_SPLIT), dict(blocks=7),
+    dict(blocks=7),
 ], ids=lambda kw: ",".join(f"{k}={v}" for k, v in kw.items()) or "default")
 def test_batch_counts_match_oracle(gpu, bctx, batch, kw):
     beg, adj, vb, want = batch

@@ -86,29 +86,10 @@ def test_rmat16_invariance(ctx16, rmat16, kw):
     assert r.probes == rmat16[3]
 
 
-@pytest.mark.parametrize("kw", [
-    dict(), dict(threads=64), dict(threads=1024), dict(hub_threshold=0), dict(blocks=5, chunk=3),
-    dict(flags=T.FLAG_FORCE_SPILL), dict(flags=T.FLAG_LINEAR_PROBING), dict(flags=T.FLAG_WAVE_ONLY),
-])
-def test_rmat16_xcd_split(ctx16, rmat16, kw):
-    # 8 walk jobs by v % 8 over class-ordered rows, job x first on XCD x
-    kw = dict(kw)
-    kw["flags"] = kw.get("flags", 0) | T.FLAG_XCD_SPLIT
-    r = ctx16.count(**kw)
-    assert r.triangles == rmat16[2]
-    assert r.probes == rmat16[3]
-    # the class-ordered rows still count correctly without the split
-    assert ctx16.count().triangles == rmat16[2]
-
-
-def test_xcd_split_ranks_and_offsets64(gpu, rmat16):
-    beg, adj, want, w = rmat16
-    with T.Context(gpu) as c:
-        c.load(beg, adj, flags=T.FLAG_OFFSETS64)
-        assert c.count(flags=T.FLAG_XCD_SPLIT).triangles == want
-        parts = [c.count(flags=T.FLAG_XCD_SPLIT, rank=r, total_process=3).triangles
-                 for r in range(3)]
-        assert sum(parts) == want
+def test_unknown_flags_rejected(ctx16):
+    for bad in (32, 128, 1 << 31):
+        with pytest.raises(T.TrustError):
+            ctx16.count(flags=bad)
 
 
 def test_rmat16_offsets64(gpu, rmat16):

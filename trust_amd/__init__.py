@@ -27,7 +27,6 @@ TRUST_OK = 0
 ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2, -3, -4, -5, -6, -7)
 SCHED_DEGREE, SCHED_LEARNED = 0, 1
 FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
-FLAG_XCD_SPLIT = 32
 FLAG_HIP_GRAPH = 64
 # launch parameters the autotuner searches
 TUNED_FIELDS = ("threads", "wave_slots", "hub_threshold", "chunk")

@@ -32,17 +32,16 @@ static std::vector<uintptr_t> capture_key(const Ctx *ctx, const trust_params &p)
 {
     std::vector<uintptr_t> k((sizeof(p) + sizeof(uintptr_t) - 1) / sizeof(uintptr_t), 0);
     memcpy(k.data(), &p, sizeof(p));
-    const void *ptrs[] = {ctx->g.beg.p,     ctx->g.adj.p,       ctx->list.p,
-                          ctx->block_counts.p, ctx->ctl.p,      ctx->spill.p,
-                          ctx->class_starts.p, ctx->feat.p,     ctx->cost.p,
-                          ctx->tiers.p,     ctx->mlp_params.p,  ctx->batch_vbase.p,
-                          ctx->batch_page.p, ctx->batch_out.p};
+    const void *ptrs[] = {ctx->g.beg.p,      ctx->g.adj.p,      ctx->list.p,
+                          ctx->block_counts.p, ctx->ctl.p,       ctx->spill.p,
+                          ctx->feat.p,       ctx->cost.p,       ctx->tiers.p,
+                          ctx->mlp_params.p, ctx->batch_vbase.p, ctx->batch_page.p,
+                          ctx->batch_out.p};
     for (const void *q : ptrs)
         k.push_back((uintptr_t)q);
     k.push_back((uintptr_t)ctx->g.V);
     k.push_back((uintptr_t)ctx->g.E);
     k.push_back((uintptr_t)ctx->batch_ngraphs);
-    k.push_back((uintptr_t)ctx->class_n);
     k.push_back((uintptr_t)ctx->mlp_ready);
     return k;
 }
@@ -159,8 +158,8 @@ extern "C" int trust_ctx_count_batch(trust_ctx *c, const trust_params *p, uint64
     }
     const int64_t ng = ctx->batch_ngraphs;
     std::vector<uint64_t> out((size_t)(2 * ng + 1));
-    TcCtl hj[kMaxJobs];
-    HIPCHK(hipMemcpyAsync(hj, ctx->ctl.p, sizeof(TcCtl) * R.nj, hipMemcpyDeviceToHost, ctx->stream));
+    TcCtl h;
+    HIPCHK(hipMemcpyAsync(&h, ctx->ctl.p, sizeof(TcCtl), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(out.data(), ctx->batch_out.p, sizeof(uint64_t) * out.size(),
                           hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -179,6 +178,6 @@ extern "C" int trust_ctx_count_batch(trust_ctx *c, const trust_params *p, uint64
             done_seconds[g] = d > t0 ? (double)(d - t0) / ctx->wall_hz : 0.0;
         }
     }
-    finish_resident(ctx, R, hj, t_plan * 1e-3, t_count * 1e-3, r);
+    finish_resident(ctx, R, h, t_plan * 1e-3, t_count * 1e-3, r);
     return TRUST_OK;
 }

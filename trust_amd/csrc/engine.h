@@ -50,16 +50,6 @@ struct CsrViews {
     const int32_t *adjP;
 };
 
-// Walk views of a multi-job count (XCD class split); row u of job j is
-// adjW[begW[u], endW[u]).
-struct JobSet {
-    int n = 0;
-    bool xcd = false;
-    const void *begW[kMaxJobs];
-    const void *endW[kMaxJobs];
-    const int32_t *adjW[kMaxJobs];
-};
-
 // Shape of one counting launch.
 struct LaunchShape {
     int threads = 0, blocks = 0;
@@ -94,9 +84,6 @@ struct Ctx {
     double h2d_seconds = 0;
     double learned_plan_seconds = 0;
     DevBuf staging, list, block_counts, ctl, spill, instr_cycles, instr_probes;
-    // XCD class split: rows of g.adj grouped by v % class_n, class starts
-    DevBuf class_starts;
-    int class_n = 0;
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     bool mlp_ready = false;
@@ -126,8 +113,7 @@ template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
-                  const JobSet *jobs = nullptr);
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count);
 
 int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
@@ -137,9 +123,6 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
 // enqueue step can also be captured into a hipGraph.
 struct ResidentRun {
     LaunchPlan L;
-    JobSet js;
-    const JobSet *jsp = nullptr;
-    int nj = 1;
     bool learned = false;
 };
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
@@ -149,7 +132,7 @@ void drop_captured(Ctx *ctx);
 // records ctx->ev[0..2] around plan and count (not under capture).
 int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRun &R, bool timed);
 // Fills *r from the control blocks copied back after the count.
-void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl *hj, double t_plan,
+void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_plan,
                      double t_count, trust_result *r);
 
 // Learned scheduler (learned.hip): enqueues features, MLP inference, tier

@@ -204,8 +204,6 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.begP = (const OffT *)views.begP;
     a.adjP = views.adjP;
     a.list = (const int32_t *)ctx->list.p;
-    a.njobs = 1;
-    a.jobs[0] = TcJob<OffT>{a.begW, a.endW, a.adjW, a.list, ctl_dev};
     a.spill_mem = (int32_t *)ctx->spill.p;
     a.spill_slots = L.spill_slots;
     a.spill_groups = L.spill_groups;
@@ -237,20 +235,11 @@ template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
-                  const JobSet *jobs)
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count)
 {
-    const int nj = jobs ? jobs->n : 1;
-    int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1) * nj);
+    int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
     if (rc)
         return rc;
-    const OffT *jbeg[kMaxJobs], *jend[kMaxJobs];
-    const int32_t *jadj[kMaxJobs];
-    for (int j = 0; j < nj; j++) {
-        jbeg[j] = (const OffT *)(jobs ? jobs->begW[j] : views.begW);
-        jend[j] = jobs ? (const OffT *)jobs->endW[j] : (const OffT *)views.begW + 1;
-        jadj[j] = jobs ? jobs->adjW[j] : views.adjW;
-    }
     const int per_block = kPlanPerBlock;
     const int nblocks = (int)((V + per_block - 1) / per_block);
     rc = ctx->block_counts.ensure(sizeof(long long) * (size_t)kNumTiers * std::max(nblocks, 1));
@@ -261,52 +250,40 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         if (rc)
             return rc;
     }
-    if (!preplanned)
-    for (int j = 0; j < nj; j++) {
-    HIPCHK(launch_fill_u32((uint32_t *)(ctl_dev + j), 0u, sizeof(TcCtl) / 4, s));
-
-    PlanArgs<OffT> pa;
-    memset(&pa, 0, sizeof(pa));
-    pa.begH = (const OffT *)views.begH;
-    pa.begW = jbeg[j];
-    pa.endW = jend[j];
-    pa.V = V;
-    pa.same_csr = same_csr ? 1 : 0;
-    pa.hub_threshold = p.hub_threshold;
-    pa.wave_lg = L.main.wave_lg;
-    pa.hub_lg = L.main.group_lg;
-    pa.big_lg = L.has_big ? L.big.group_lg : 0;
-    pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
-    pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
-    pa.tier_override = nullptr;
-    pa.list = (int32_t *)ctx->list.p + (size_t)j * std::max<int64_t>(V, 1);
-    pa.block_counts = (long long *)ctx->block_counts.p;
-    pa.per_block = per_block;
-    pa.ctl = ctl_dev + j;
-    HIPCHK(launch_plan<OffT>(pa, nblocks, s));
+    if (!preplanned) {
+        HIPCHK(launch_fill_u32((uint32_t *)ctl_dev, 0u, sizeof(TcCtl) / 4, s));
+        PlanArgs<OffT> pa;
+        memset(&pa, 0, sizeof(pa));
+        pa.begH = (const OffT *)views.begH;
+        pa.begW = (const OffT *)views.begW;
+        pa.endW = (const OffT *)views.begW + 1;
+        pa.V = V;
+        pa.same_csr = same_csr ? 1 : 0;
+        pa.hub_threshold = p.hub_threshold;
+        pa.wave_lg = L.main.wave_lg;
+        pa.hub_lg = L.main.group_lg;
+        pa.big_lg = L.has_big ? L.big.group_lg : 0;
+        pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
+        pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
+        pa.tier_override = nullptr;
+        pa.list = (int32_t *)ctx->list.p;
+        pa.block_counts = (long long *)ctx->block_counts.p;
+        pa.per_block = per_block;
+        pa.ctl = ctl_dev;
+        HIPCHK(launch_plan<OffT>(pa, nblocks, s));
     }
     if (ev_plan)
         HIPCHK(hipEventRecord(ev_plan, s));
 
     TcArgs<OffT> a;
-    auto set_jobs = [&](TcArgs<OffT> &x) {
-        x.njobs = nj;
-        x.xcd_jobs = jobs && jobs->xcd ? 1 : 0;
-        for (int j = 0; j < nj; j++)
-            x.jobs[j] = TcJob<OffT>{jbeg[j], jend[j], jadj[j],
-                                    (const int32_t *)ctx->list.p + (size_t)j * std::max<int64_t>(V, 1),
-                                    ctl_dev + j};
-    };
     if (L.has_big) {
         fill_args(a, ctx, views, p, L, L.big, ctl_dev, instr_cycles, instr_probes);
-        set_jobs(a);
         a.spill_tier = L.spill_groups > 0 ? kTierSpill : -1;
         a.group_tier = kTierBig;
         a.wave_tier = -1;
         HIPCHK(launch_count<OffT>(a, L.big.blocks, L.big.threads, L.big.lds_bytes, mode, s));
     }
     fill_args(a, ctx, views, p, L, L.main, ctl_dev, instr_cycles, instr_probes);
-    set_jobs(a);
     a.spill_tier = -1;
     a.group_tier = kTierHub;
     a.wave_tier = kTierWave;
@@ -319,11 +296,11 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
 template int enqueue_count<int32_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t, const JobSet *);
+                                    hipEvent_t, hipEvent_t);
 template int enqueue_count<int64_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t, const JobSet *);
+                                    hipEvent_t, hipEvent_t);
 
 int check_params(const trust_params &p)
 {
@@ -336,6 +313,10 @@ int check_params(const trust_params &p)
         return fail(TRUST_ERR_ARG, "negative launch parameter");
     if (p.schedule != TRUST_SCHED_DEGREE && p.schedule != TRUST_SCHED_LEARNED)
         return fail(TRUST_ERR_ARG, "unknown schedule %d", p.schedule);
+    const uint32_t known = TRUST_FLAG_FORCE_SPILL | TRUST_FLAG_OFFSETS64 | TRUST_FLAG_NO_VALIDATE |
+                           TRUST_FLAG_WAVE_ONLY | TRUST_FLAG_LINEAR_PROBING | TRUST_FLAG_HIP_GRAPH;
+    if (p.flags & ~known)
+        return fail(TRUST_ERR_ARG, "unknown flag bits 0x%x", p.flags & ~known);
     return TRUST_OK;
 }
 
@@ -344,37 +325,6 @@ double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
     // offsets once + every N+(u) once + two offsets per walked v + every probe
     const double ob = off64 ? 8.0 : 4.0;
     return ob * (double)(V + 1) + 4.0 * (double)E + 2.0 * ob * (double)E + 4.0 * (double)W;
-}
-
-// Rows of the resident graph grouped by v % C, with per-class row starts
-// (the XCD class split).  Any row order is a valid oriented CSR, so the
-// regrouped graph serves every later count.
-static int ensure_class_layout(Ctx *ctx, int C)
-{
-    DevCsr &g = ctx->g;
-    if (ctx->class_n == C)
-        return TRUST_OK;
-    const size_t ob = g.off64 ? 8 : 4;
-    DevBuf out;
-    int rc = out.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(g.E, 1));
-    if (!rc)
-        rc = ctx->class_starts.ensure(ob * (size_t)(C - 1) * (size_t)std::max<int64_t>(g.V, 1));
-    if (rc)
-        return rc;
-    const int blocks = ctx->num_cus * 16;
-    if (g.off64)
-        HIPCHK(launch_class_layout<int64_t>((const int64_t *)g.beg.p, (const int32_t *)g.adj.p,
-                                            (int32_t *)out.p, g.V, C,
-                                            (int64_t *)ctx->class_starts.p, blocks, ctx->stream));
-    else
-        HIPCHK(launch_class_layout<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
-                                            (int32_t *)out.p, g.V, C,
-                                            (int32_t *)ctx->class_starts.p, blocks, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    std::swap(g.adj.p, out.p);
-    std::swap(g.adj.bytes, out.bytes);
-    ctx->class_n = C;
-    return TRUST_OK;
 }
 
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
@@ -393,27 +343,7 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
     rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, R.L, mode);
     if (rc)
         return rc;
-    // XCD class split: job x walks the v with v % 8 == x, on XCD x first
-    R.jsp = nullptr;
-    if ((p.flags & TRUST_FLAG_XCD_SPLIT) && g.V > 0) {
-        const int C = kMaxJobs;
-        rc = ensure_class_layout(ctx, C);
-        if (rc)
-            return rc;
-        const size_t ob = g.off64 ? 8 : 4;
-        const char *starts = (const char *)ctx->class_starts.p;
-        R.js.n = C;
-        R.js.xcd = true;
-        for (int x = 0; x < C; x++) {
-            R.js.begW[x] = x == 0 ? g.beg.p : starts + ob * (size_t)(x - 1) * g.V;
-            R.js.endW[x] = x == C - 1 ? (const void *)((const char *)g.beg.p + ob)
-                                      : starts + ob * (size_t)x * g.V;
-            R.js.adjW[x] = (const int32_t *)g.adj.p;
-        }
-        R.jsp = &R.js;
-    }
-    R.nj = R.jsp ? R.jsp->n : 1;
-    rc = ctx->ctl.ensure(sizeof(TcCtl) * R.nj);
+    rc = ctx->ctl.ensure(sizeof(TcCtl));
     if (rc)
         return rc;
     if (mode == kModeInstr) {
@@ -425,8 +355,7 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
             return rc;
     }
     // learned schedule when a cost model is loaded, else the degree split
-    // (the learned queue is single-job: with the XCD split it is not used)
-    R.learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready && !R.jsp;
+    R.learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready;
     return TRUST_OK;
 }
 
@@ -459,16 +388,16 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
         rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, R.learned,
-                                    ctx->stream, e1, e2, R.jsp);
+                                    ctx->stream, e1, e2);
     else
         rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, mode,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, R.learned,
-                                    ctx->stream, e1, e2, R.jsp);
+                                    ctx->stream, e1, e2);
     return rc;
 }
 
-void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl *hj, double t_plan,
+void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_plan,
                      double t_count, trust_result *r)
 {
     if (!r)
@@ -478,19 +407,17 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl *hj, double t_p
     memset(r, 0, sizeof(*r));
     r->vertices = g.V;
     r->edges = g.E;
-    r->triangles = hj[0].count;
-    r->probes = hj[0].probes;
+    r->triangles = h.count;
+    r->probes = h.probes;
     r->plan_seconds = t_plan;
     r->count_seconds = t_count;
     r->seconds = t_plan + t_count;
     r->h2d_seconds = ctx->h2d_seconds;
     r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
-    r->alg_bytes = alg_bytes(g.V, g.E, hj[0].probes, g.off64);
-    for (int j = 0; j < R.nj; j++) {  // vertex-job pairs per tier
-        r->n_spill += hj[j].list_begin[kTierSpill + 1] - hj[j].list_begin[kTierSpill];
-        r->n_hub += hj[j].list_begin[kTierHub + 1] - hj[j].list_begin[kTierBig];  // big + hub
-        r->n_wave += hj[j].list_begin[kTierWave + 1] - hj[j].list_begin[kTierWave];
-    }
+    r->alg_bytes = alg_bytes(g.V, g.E, h.probes, g.off64);
+    r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
+    r->n_hub = h.list_begin[kTierHub + 1] - h.list_begin[kTierBig];  // big + hub
+    r->n_wave = h.list_begin[kTierWave + 1] - h.list_begin[kTierWave];
     r->threads = L.main.threads;
     r->blocks = L.main.blocks;
     r->hub_slots = 1 << L.main.group_lg;
@@ -511,8 +438,8 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
     if (rc)
         return rc;
     const DevCsr &g = ctx->g;
-    TcCtl hj[kMaxJobs];
-    HIPCHK(hipMemcpyAsync(hj, ctx->ctl.p, sizeof(TcCtl) * R.nj, hipMemcpyDeviceToHost, ctx->stream));
+    TcCtl h;
+    HIPCHK(hipMemcpyAsync(&h, ctx->ctl.p, sizeof(TcCtl), hipMemcpyDeviceToHost, ctx->stream));
     if (mode == kModeInstr) {
         if (cycles_out)
             HIPCHK(hipMemcpyAsync(cycles_out, ctx->instr_cycles.p,
@@ -527,7 +454,7 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
     float t_plan = 0, t_count = 0;
     HIPCHK(hipEventElapsedTime(&t_plan, ctx->ev[0], ctx->ev[1]));
     HIPCHK(hipEventElapsedTime(&t_count, ctx->ev[1], ctx->ev[2]));
-    finish_resident(ctx, R, hj, t_plan * 1e-3, t_count * 1e-3, r);
+    finish_resident(ctx, R, h, t_plan * 1e-3, t_count * 1e-3, r);
     return TRUST_OK;
 }
 
@@ -582,7 +509,6 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
     HIPCHK(hipStreamSynchronize(ctx->stream));
     // state derived from the previous graph
     drop_captured(ctx);
-    ctx->class_n = 0;
     ctx->batch_ngraphs = 0;
     auto t0 = std::chrono::steady_clock::now();
     int rc = upload_csr(ctx, ctx->g, beg, adj, V, E, (flags & TRUST_FLAG_OFFSETS64) != 0,

@@ -37,36 +37,19 @@ struct TcCtl {
     unsigned long long pad[4];
 };
 
-// One walk view with its own work list and control block.  Row u of the
-// walk is adjW[begW[u], endW[u]) (endW = begW + 1 for a plain CSR; for an
-// XCD class slice both point into the class-ordered rows).
-template <typename OffT>
-struct TcJob {
-    const OffT *begW;
-    const OffT *endW;
-    const int32_t *adjW;
-    const int32_t *list;
-    TcCtl *ctl;
-};
-constexpr int kMaxJobs = 8;
-
 // Three CSR views: the hash source (N+(u) that is tabled), the walk (the v
 // in N+(u) whose lists are probed) and the probe rows (N+(v)).  For an
 // unpartitioned graph all three are the same arrays; for a 2D tile job they
-// are tiles (i,j), (i,k) and (k,j); for an XCD class split, job x walks only
-// the v with v % njobs == x and runs on XCD x first.
+// are tiles (i,j), (i,k) and (k,j).
 template <typename OffT>
 struct TcArgs {
     const OffT *begH;
     const int32_t *adjH;
-    const OffT *begW;        // current job's walk (set by the kernel from jobs[])
+    const OffT *begW;        // row u of the walk is adjW[begW[u], endW[u])
     const OffT *endW;
     const int32_t *adjW;
     const OffT *begP;
     const int32_t *adjP;
-    TcJob<OffT> jobs[kMaxJobs];
-    int njobs;
-    int xcd_jobs;            // 1: a workgroup starts with job (hardware XCC id % njobs)
     const int32_t *list;     // vertices grouped by tier (see TcCtl::list_begin)
     int32_t *spill_mem;      // HBM hash tables, spill_slots ints per workgroup
     long long spill_slots;
@@ -127,9 +110,6 @@ hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t l
 template <typename OffT>
 hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, int *blocks_per_cu);
 hipError_t launch_copy(const int4 *src, int4 *dst, long long n, int blocks, hipStream_t stream);
-template <typename OffT>
-hipError_t launch_class_layout(const OffT *beg, const int32_t *adj, int32_t *out, long long V,
-                               int C, OffT *starts, int blocks, hipStream_t stream);
 hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
                                  hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream_t stream);

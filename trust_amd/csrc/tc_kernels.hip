@@ -98,58 +98,58 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
     return x;
 }
 
-__device__ __forceinline__ long long uniform(long long x)
-{
-    const int lo = __builtin_amdgcn_readfirstlane((int)(x & 0xffffffff));
-    const int hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
-    return ((long long)hi << 32) | (unsigned int)lo;
-}
-
-// Per-wave attribution of triangles to the graphs of a batch: the wave
-// keeps the vertex range of its current graph and per-lane running sums,
-// and adds their total (with a completion time stamp) when it moves to
-// another graph.  Called with the whole wave active and u wave-uniform.
+// Per-wave attribution of triangles to the graphs of a batch.  The wave
+// keeps the vertex range of its current graph and the wave total of its
+// running count when it entered that graph, all wave-uniform (scalar
+// registers).  Entering a vertex of another graph adds the difference to the
+// previous graph, with a completion time stamp.  Called with the whole wave
+// active, u wave-uniform, before the vertex's hits are added to acc.
 struct GraphAcc {
     int g = -1;
-    long long lo = 0, hi = 0;
-    unsigned long long sum = 0;  // this lane's share
+    int lo = 0, hi = 0;
+    unsigned long long base = 0;  // wave total of acc when g was entered
 };
 
-template <typename Args>
-__device__ __forceinline__ void graph_flush(const Args &a, GraphAcc &ga)
+__device__ __forceinline__ unsigned long long uniform(unsigned long long x)
 {
-    const unsigned long long s = wave_sum(ga.sum);
-    if (ga.g >= 0 && __lane_id() == 0) {
-        if (s)
-            atomicAdd(a.graph_counts + ga.g, s);
-        atomicMax(a.graph_done + ga.g, (unsigned long long)wall_clock64());
-    }
-    ga.g = -1;
-    ga.sum = 0;
+    const int lo = __builtin_amdgcn_readfirstlane((int)(x & 0xffffffffu));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
 }
 
 template <typename Args>
-__device__ __forceinline__ void graph_add(const Args &a, GraphAcc &ga, long long u,
-                                          unsigned int lane_hits)
+__device__ __forceinline__ void graph_flush(const Args &a, GraphAcc &ga, unsigned long long acc)
 {
-    if (u < ga.lo || u >= ga.hi || ga.g < 0) {
-        graph_flush(a, ga);
-        // the page of u bounds the search: vbase[lo] <= u < vbase[hi]
-        const long long pg = u >> a.graph_page_shift;
-        int lo = a.graph_page[pg], hi = a.graph_page[pg + 1] + 1;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (a.graph_vbase[mid] <= u)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        // wave-uniform values: keep them in scalar registers
-        ga.g = __builtin_amdgcn_readfirstlane(lo);
-        ga.lo = uniform(a.graph_vbase[ga.g]);
-        ga.hi = uniform(a.graph_vbase[ga.g + 1]);
+    const unsigned long long total = uniform(wave_sum(acc));
+    if (ga.g >= 0 && __lane_id() == 0) {
+        if (total != ga.base)
+            atomicAdd(a.graph_counts + ga.g, total - ga.base);
+        atomicMax(a.graph_done + ga.g, (unsigned long long)wall_clock64());
     }
-    ga.sum += lane_hits;
+    ga.base = total;
+    ga.g = -1;
+}
+
+template <typename Args>
+__device__ __forceinline__ void graph_enter(const Args &a, GraphAcc &ga, int u,
+                                            unsigned long long acc)
+{
+    if (ga.g >= 0 && u >= ga.lo && u < ga.hi)
+        return;
+    graph_flush(a, ga, acc);
+    // the page of u bounds the search: vbase[lo] <= u < vbase[hi]
+    const int pg = u >> a.graph_page_shift;
+    int lo = a.graph_page[pg], hi = a.graph_page[pg + 1] + 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.graph_vbase[mid] <= u)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    ga.g = __builtin_amdgcn_readfirstlane(lo);
+    ga.lo = __builtin_amdgcn_readfirstlane((int)a.graph_vbase[ga.g]);
+    ga.hi = __builtin_amdgcn_readfirstlane((int)a.graph_vbase[ga.g + 1]);
 }
 
 // ---------------------------------------------------------- hash tables
@@ -223,7 +223,10 @@ __device__ __forceinline__ bool cuckoo_insert(int *tab, int lgb, int key)
     return true;
 }
 
-// Membership count of up to kUnroll keys (key < 0: not a probe).
+// Membership count of up to kUnroll keys (key < 0: not a probe).  Both
+// buckets of every key are read at once: reading the second bucket only
+// when the first is full (sound, as slots are never vacated) saves LDS reads
+// but serialises two LDS round trips, and measured 12% slower at R-MAT 24.
 template <bool kGlobal>
 __device__ __forceinline__ unsigned int cuckoo_count(const int *tab, int lg, const int (&w)[kUnroll])
 {
@@ -429,6 +432,8 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
 {
     constexpr bool kInstr = kMode == kModeInstr;
     constexpr bool kTable = kMode != kModeGather;
+    if constexpr (kMode == kModeBatch)
+        graph_enter(a, ga, u, acc);
     unsigned long long t0 = 0;
     if (kInstr && threadIdx.x == 0)
         t0 = clock64();
@@ -562,8 +567,6 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
     }
     acc += hits;
     probes += wave_probes;
-    if constexpr (kMode == kModeBatch)
-        graph_add(a, ga, u, hits);
     __syncthreads();
     if constexpr (kInstr) {
         if (lane == 0 && wave_probes)
@@ -581,6 +584,8 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
 {
     constexpr bool kInstr = kMode == kModeInstr;
     constexpr bool kTable = kMode != kModeGather;
+    if constexpr (kMode == kModeBatch)
+        graph_enter(a, ga, u, acc);
     unsigned long long t0 = 0;
     if (kInstr)
         t0 = clock64();
@@ -623,8 +628,6 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     wave_sync();
     acc += hits;
     probes += wave_probes;
-    if constexpr (kMode == kModeBatch)
-        graph_add(a, ga, u, hits);
     if constexpr (kInstr) {
         if (lane == 0) {
             a.instr_probes[u] = wave_probes;
@@ -659,8 +662,13 @@ __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int
     }
 }
 
+// Eight waves per SIMD: the compiler fits the kernel in 64 VGPRs (a few
+// spills outside the probe loop).  The walk is latency-bound, and the
+// eighth wave measured 15% faster at R-MAT 24 than seven waves without
+// spills (1.6x at 1024-thread workgroups).
 template <typename OffT, int kMode>
-__global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+tc_count_kernel(TcArgs<OffT> a)
 {
     extern __shared__ int lds[];
     const int nwaves = blockDim.x / kWave;
@@ -675,61 +683,45 @@ __global__ void __launch_bounds__(1024) tc_count_kernel(TcArgs<OffT> a)
     if (kMode == kModeBatch && threadIdx.x == 0)
         atomicMin(a.graph_t0, (unsigned long long)wall_clock64());
 
-    // Job order: with an XCD class split, start with the job of this
-    // workgroup's XCD (hardware XCC_ID) so each XCD's L2 serves one class of
-    // probe lists; then help the other jobs until every queue is empty.
-    int first_job = 0;
-    if (a.xcd_jobs && a.njobs > 1)
-        first_job = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) % (unsigned)a.njobs);  // XCC_ID[3:0]
-    for (int jj = 0; jj < a.njobs; jj++) {
-        TcArgs<OffT> b = a;
-        const TcJob<OffT> &J = a.jobs[(first_job + jj) % a.njobs];
-        b.begW = J.begW;
-        b.endW = J.endW;
-        b.adjW = J.adjW;
-        b.list = J.list;
-        b.ctl = J.ctl;
-        if (b.spill_tier >= 0 && (int)blockIdx.x < b.spill_groups) {
-            int *tab = b.spill_mem + (long long)blockIdx.x * b.spill_slots;
-            int cap_lg = 0;
-            while ((2ll << cap_lg) <= b.spill_slots)
-                cap_lg++;
-            group_phase<OffT, kMode, true>(b, b.spill_tier, tab, cap_lg, mark, misc, ctot, acc,
-                                           probes, ga);
+    if (a.spill_tier >= 0 && (int)blockIdx.x < a.spill_groups) {
+        int *tab = a.spill_mem + (long long)blockIdx.x * a.spill_slots;
+        int cap_lg = 0;
+        while ((2ll << cap_lg) <= a.spill_slots)
+            cap_lg++;
+        group_phase<OffT, kMode, true>(a, a.spill_tier, tab, cap_lg, mark, misc, ctot, acc, probes,
+                                       ga);
+    }
+    if (a.group_tier >= 0)
+        group_phase<OffT, kMode, false>(a, a.group_tier, lds, a.group_lg, mark, misc, ctot, acc,
+                                        probes, ga);
+    __syncthreads();  // the wave tables reuse the group table's LDS
+    if (a.wave_tier >= 0) {
+        const long long lb = a.ctl->list_begin[a.wave_tier];
+        const long long le = a.ctl->list_begin[a.wave_tier + 1];
+        int *wtab = lds + wave * a.wave_slots;
+        for (;;) {
+            unsigned int k = 0;
+            if (lane == 0)
+                k = atomicAdd(&a.ctl->next[a.wave_tier], 1u);
+            k = __builtin_amdgcn_readfirstlane(k);
+            const long long c = (long long)k * a.nranks + a.rank;
+            const long long first = lb + c * a.chunk_wave;
+            if (first >= le)
+                break;
+            const long long last = min(first + (long long)a.chunk_wave, le);
+            for (long long i = first; i < last; i++)
+                wave_vertex<OffT, kMode>(a, a.list[i], wtab, mark, acc, probes, ga);
         }
-        if (b.group_tier >= 0)
-            group_phase<OffT, kMode, false>(b, b.group_tier, lds, b.group_lg, mark, misc, ctot,
-                                            acc, probes, ga);
-        __syncthreads();
-        if (b.wave_tier >= 0) {
-            const long long lb = b.ctl->list_begin[b.wave_tier];
-            const long long le = b.ctl->list_begin[b.wave_tier + 1];
-            int *wtab = lds + wave * b.wave_slots;
-            for (;;) {
-                unsigned int k = 0;
-                if (lane == 0)
-                    k = atomicAdd(&b.ctl->next[b.wave_tier], 1u);
-                k = __builtin_amdgcn_readfirstlane(k);
-                const long long c = (long long)k * b.nranks + b.rank;
-                const long long first = lb + c * b.chunk_wave;
-                if (first >= le)
-                    break;
-                const long long last = min(first + (long long)b.chunk_wave, le);
-                for (long long i = first; i < last; i++)
-                    wave_vertex<OffT, kMode>(b, b.list[i], wtab, mark, acc, probes, ga);
-            }
-        }
-        __syncthreads();
     }
     if constexpr (kMode == kModeBatch)
-        graph_flush(a, ga);
+        graph_flush(a, ga, acc);
     acc = wave_sum(acc);
     probes = wave_sum(probes) / kWave;  // every lane carried the wave's probe total
-    if (lane == 0) {  // totals accumulate in job 0's control block
+    if (lane == 0) {
         if (acc)
-            atomicAdd(&a.jobs[0].ctl->count, acc);
+            atomicAdd(&a.ctl->count, acc);
         if (probes)
-            atomicAdd(&a.jobs[0].ctl->probes, probes);
+            atomicAdd(&a.ctl->probes, probes);
     }
 }
 
@@ -866,74 +858,6 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
         dst[i] = (int32_t)src[i];
 }
 
-// Class-ordered rows for the XCD split: row u of `out` holds the entries of
-// row u of `adj` grouped by v % C (stable inside a class); starts[(x-1)*V+u]
-// is where class x begins in row u (class 0 begins at beg[u], class C ends
-// at beg[u+1]).  One wave per row.
-template <typename OffT>
-__global__ void __launch_bounds__(256) class_layout_kernel(const OffT *__restrict__ beg,
-                                                           const int32_t *__restrict__ adj,
-                                                           int32_t *__restrict__ out, long long V,
-                                                           int C, OffT *__restrict__ starts)
-{
-    const long long waves = (long long)gridDim.x * (blockDim.x / kWave);
-    const int lane = lane_id();
-    for (long long u = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < V;
-         u += waves) {
-        const OffT s = beg[u];
-        const int d = (int)(beg[u + 1] - s);
-        int cnt[kMaxJobs];
-#pragma unroll
-        for (int c = 0; c < kMaxJobs; c++)
-            cnt[c] = 0;
-        for (int i = 0; i < d; i += kWave) {
-            const bool valid = i + lane < d;
-            const int cls = valid ? (int)((unsigned int)adj[s + (OffT)(i + lane)] % (unsigned)C) : -1;
-#pragma unroll
-            for (int c = 0; c < kMaxJobs; c++)
-                cnt[c] += (int)__popcll(__ballot(cls == c));
-        }
-        OffT off[kMaxJobs];
-        OffT run = s;
-#pragma unroll
-        for (int c = 0; c < kMaxJobs; c++) {
-            off[c] = run;
-            run += (OffT)cnt[c];
-            if (c >= 1 && c < C && lane == 0)
-                starts[(long long)(c - 1) * V + u] = off[c];
-        }
-        for (int i = 0; i < d; i += kWave) {
-            const bool valid = i + lane < d;
-            const int val = valid ? adj[s + (OffT)(i + lane)] : 0;
-            const int cls = valid ? (int)((unsigned int)val % (unsigned)C) : -1;
-#pragma unroll
-            for (int c = 0; c < kMaxJobs; c++) {
-                const unsigned long long m = __ballot(cls == c);
-                if (cls == c) {
-                    const int r = (int)__builtin_amdgcn_mbcnt_hi(
-                        (unsigned int)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0));
-                    out[off[c] + (OffT)r] = val;
-                }
-                off[c] += (OffT)__popcll(m);
-            }
-        }
-    }
-}
-
-template <typename OffT>
-hipError_t launch_class_layout(const OffT *beg, const int32_t *adj, int32_t *out, long long V,
-                               int C, OffT *starts, int blocks, hipStream_t stream)
-{
-    if (V <= 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(class_layout_kernel<OffT>, dim3(blocks), dim3(256), 0, stream, beg, adj, out,
-                       V, C, starts);
-    return hipGetLastError();
-}
-template hipError_t launch_class_layout<int32_t>(const int32_t *, const int32_t *, int32_t *,
-                                                 long long, int, int32_t *, int, hipStream_t);
-template hipError_t launch_class_layout<int64_t>(const int64_t *, const int32_t *, int32_t *,
-                                                 long long, int, int64_t *, int, hipStream_t);
 
 // Plain streaming copy: the known-traffic kernel that calibrates HBM byte
 // counters and gives the sequential-bandwidth reference.
