@@ -338,6 +338,13 @@ def bench_batch(args, T):
         f"{E / dev_s / 1e9:.3f} GTEPS, latency p50 {p50 * 1e3:.2f} ms p99 {p99 * 1e3:.2f} ms")
     eager = run_steps(lambda: ctx.count_batch(**kw)[2], 0, max(1, args.steps // 2),
                       "eager batch", E)
+    # the plain degree split on the same batch, same launch parameters
+    dkw = dict(kw, schedule=T.SCHED_DEGREE, flags=T.FLAG_HIP_GRAPH)
+    ctx.count_batch(**dkw)
+    deg = run_steps(lambda: ctx.count_batch(**dkw)[2], 0, max(1, args.steps // 2),
+                    "degree split, hipGraph", E)
+    _, ddone, _ = ctx.count_batch(**dkw)
+    dp50, dp99 = np.percentile(ddone, [50, 99])
     workload = f"batch{ng}_er_ba_n{args.nmin}-{args.nmax}_d8-64_seed{args.seed}"
     line = line_head(args, "batch", ng / dev_s, "graphs/s", dev_s * 1e3)
     line["ms_per_step_wall"] = wall * 1e3
@@ -350,6 +357,10 @@ def bench_batch(args, T):
         "latency_definition": "per graph: first workgroup start to the end of the graph's "
                               "last vertex, device wall clock, pooled over the timed replays",
         "eager_ms_per_batch": eager["dev_s"] * 1e3,
+        "eager_plan_ms": (eager["dev_s"] - eager["cnt_s"]) * 1e3,
+        "degree_split_graphs_per_s": ng / deg["dev_s"],
+        "degree_split_latency_p50_ms": dp50 * 1e3, "degree_split_latency_p99_ms": dp99 * 1e3,
+        "learned_vs_degree_split": deg["dev_s"] / dev_s,
         "per_graph_parity": f"all {ng} graphs equal the oracle",
         "cost_model_training": dict(mse=mse, seconds=train_s),
         "autotune": tune,

@@ -151,3 +151,21 @@ def test_load_batch_rejects_bad_layout(gpu):
             c.load_batch(beg, cross, vb)
         with pytest.raises(T.TrustError):
             c.count_batch()  # nothing loaded
+
+
+@pytest.mark.parametrize("schedule", [T.SCHED_DEGREE, T.SCHED_LEARNED])
+def test_graphs_complete_progressively(gpu, schedule):
+    # graphs are drained in vertex-range order (the learned queue by
+    # superblocks, longest-first inside each), so half of the graphs finish
+    # well before the last one
+    beg, adj, vb = T.batch_generate(2000, seed=5, nmin=300, nmax=3000, dmin=8, dmax=32)
+    with T.Context(gpu) as c:
+        c.load_batch(beg, adj, vb)
+        if schedule == T.SCHED_LEARNED:
+            c.train_cost_model(epochs=5, max_samples=50000, seed=1)
+        for _ in range(2):
+            counts, done, r = c.count_batch(schedule=schedule, flags=T.FLAG_HIP_GRAPH)
+        assert r.schedule_used == schedule
+        p50, p100 = np.percentile(done, 50), done.max()
+        print(f"[gpu] schedule {schedule}: p50 {p50 * 1e3:.3f} ms, max {p100 * 1e3:.3f} ms", flush=True)
+        assert p50 < 0.8 * p100

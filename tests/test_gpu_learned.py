@@ -75,6 +75,27 @@ def test_features_match_oracle(gpu, rmat16):
     assert np.allclose(f, want, rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("case", ["K150", "union", "tiny_graphs", "rmat12", "K300"])
+def test_features_edge_cases(gpu, case):
+    # lists longer than a window span, empty lists, V not a multiple of 64
+    from conftest import closed_form_csr
+    if case in ("K150", "union"):
+        beg, adj, _ = closed_form_csr(case)
+    elif case == "tiny_graphs":
+        rng = np.random.default_rng(2)
+        beg, adj = T.preprocess_edges(rng.integers(0, 5000, 3000).astype(np.int32),
+                                      rng.integers(0, 5000, 3000).astype(np.int32))
+    elif case == "rmat12":
+        beg, adj = T.rmat_csr(12, 8, seed=7)
+    else:  # transitive tournament: out-lists of up to 299 entries (> 4 windows)
+        i, j = np.triu_indices(300, 1)
+        beg, adj = T.preprocess_edges(i.astype(np.int32), j.astype(np.int32))
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        f = c.features()
+    assert np.allclose(f, oracle.mlp_scaled_features(beg, adj), rtol=1e-5, atol=1e-6), case
+
+
 def test_learned_without_model_falls_back(gpu, rmat16):
     beg, adj, want = rmat16
     with T.Context(gpu) as c:

@@ -148,9 +148,11 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
     if (p.chunk > 0) {
         L.chunk_group = L.chunk_wave = p.chunk;
     } else {
+        // about 16 grabs per wave, at most 32 vertices each (large chunks of
+        // a skewed list leave long chunks for the end)
         L.chunk_group = 1;
-        long long waves = (long long)m.blocks * nwaves;
-        L.chunk_wave = (int)std::max<long long>(1, std::min<long long>(16, V / (waves * 64 + 1)));
+        const long long waves = (long long)m.blocks * nwaves;
+        L.chunk_wave = (int)std::max<long long>(1, std::min<long long>(32, V / (waves * 16 + 1)));
     }
     // large-LDS launch for tables the main launch cannot hold, plus the
     // HBM-table tier for tables no LDS can hold

@@ -49,35 +49,95 @@ __device__ __forceinline__ int wmax_i(int x)
     return x;
 }
 
-// One wave per vertex: d+(u), sum and max of d+(v), log-scaled.
+// d+(u), sum and max of d+(v) over v in N+(u), log-scaled.  A wave takes 64
+// consecutive vertices; their lists are contiguous in adj, so the wave
+// walks the concatenation in coalesced 64-wide windows, kFeatWin windows
+// at a time (their loads in flight together).  The vertex owning each
+// position comes from a mark array of list starts and a running max; each
+// lane adds d+(v) of its position into the owner's LDS slot.
+constexpr int kFeatWin = 4;
+
+__device__ __forceinline__ int fw_scan_max(int x)  // inclusive prefix max, x >= 0
+{
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+
+__device__ __forceinline__ void fw_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename OffT>
 __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ beg,
                                                        const int32_t *__restrict__ adj, long long V,
                                                        float *__restrict__ feat)
 {
+    __shared__ int mark_s[4][64 * kFeatWin];
+    __shared__ unsigned long long sum_s[4][64];
+    __shared__ int max_s[4][64];
+    const int wave = threadIdx.x / 64, lane = lane64();
+    int *mark = mark_s[wave];
+    unsigned long long *lsum = sum_s[wave];
+    int *lmax = max_s[wave];
     const long long waves = (long long)gridDim.x * (blockDim.x / 64);
-    const int lane = lane64();
-    for (long long u = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; u < V;
-         u += waves) {
-        const OffT s = beg[u];
-        const int d = (int)(beg[u + 1] - s);
-        unsigned long long sum = 0;
-        int mx = 0;
-        for (int c = 0; c < d; c += 64) {
-            if (c + lane < d) {
-                const int v = adj[s + (OffT)(c + lane)];
-                const int dv = (int)(beg[v + 1] - beg[v]);
-                sum += (unsigned long long)dv;
-                mx = max(mx, dv);
+    for (long long u0 = ((long long)blockIdx.x * (blockDim.x / 64) + wave) * 64; u0 < V;
+         u0 += waves * 64) {
+        const int n = (int)min(64ll, V - u0);
+        // lane i < n holds vertex u0+i; lanes >= n hold the end of the range
+        const OffT s = beg[u0 + min(lane, n)];
+        const OffT d = lane < n ? beg[u0 + lane + 1] - s : 0;
+        const long long e0 = (long long)__shfl(s, 0);
+        const long long e1 = (long long)__shfl(s, n == 64 ? 63 : n) + (n == 64 ? (long long)__shfl(d, 63) : 0);
+        lsum[lane] = 0;
+        lmax[lane] = 0;
+        int carry = 0;
+        for (long long p = e0; p < e1; p += 64 * kFeatWin) {
+#pragma unroll
+            for (int k = 0; k < kFeatWin; k++)
+                mark[k * 64 + lane] = 0;
+            fw_sync();
+            // list starts inside this span (an empty list starts where the
+            // next one does, so only non-empty lists mark)
+            if (d > 0 && (long long)s >= p && (long long)s < p + 64 * kFeatWin)
+                mark[(int)((long long)s - p)] = lane;
+            fw_sync();
+            int own[kFeatWin], v[kFeatWin];
+#pragma unroll
+            for (int k = 0; k < kFeatWin; k++) {
+                own[k] = max(fw_scan_max(mark[k * 64 + lane]), carry);
+                carry = __shfl(own[k], 63);
+                const long long pos = p + k * 64 + lane;
+                v[k] = pos < e1 ? adj[pos] : -1;
             }
+            int dv[kFeatWin];
+#pragma unroll
+            for (int k = 0; k < kFeatWin; k++)
+                dv[k] = v[k] >= 0 ? (int)(beg[v[k] + 1] - beg[v[k]]) : 0;
+#pragma unroll
+            for (int k = 0; k < kFeatWin; k++) {
+                if (v[k] >= 0) {
+                    atomicAdd(&lsum[own[k]], (unsigned long long)dv[k]);
+                    atomicMax(&lmax[own[k]], dv[k]);
+                }
+            }
+            fw_sync();
         }
-        sum = wsum_u64(sum);
-        mx = wmax_i(mx);
-        if (lane == 0) {
+        fw_sync();
+        if (lane < n) {
+            const long long u = u0 + lane;
             feat[3 * u] = log2f(1.f + (float)d) * kFeatScale;
-            feat[3 * u + 1] = log2f(1.f + (float)sum) * kFeatScale;
-            feat[3 * u + 2] = log2f(1.f + (float)mx) * kFeatScale;
+            feat[3 * u + 1] = log2f(1.f + (float)lsum[lane]) * kFeatScale;
+            feat[3 * u + 2] = log2f(1.f + (float)lmax[lane]) * kFeatScale;
         }
+        fw_sync();
     }
 }
 
@@ -134,15 +194,18 @@ __global__ void __launch_bounds__(256) cost_kernel(const OffT *__restrict__ beg,
         block_sum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-// total = sum of block partial sums (fixed order), one thread.
+// total = sum of block partial sums, one wave: lane i adds blocks i, i+64,
+// ... in order, then a fixed butterfly (the result does not depend on
+// timing).
 __global__ void sum_blocks_kernel(const double *block_sum, int nb, double *total)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double a = 0;
-        for (int b = 0; b < nb; b++)
-            a += block_sum[b];
+    double a = 0;
+    for (int b = threadIdx.x; b < nb; b += 64)
+        a += block_sum[b];
+    for (int off = 32; off > 0; off >>= 1)
+        a += __shfl_xor(a, off);
+    if (threadIdx.x == 0)
         *total = a;
-    }
 }
 
 struct KeyArgs {
@@ -206,8 +269,29 @@ __global__ void __launch_bounds__(256) key_hist_kernel(const unsigned short *__r
         counts[(long long)i * nb + blockIdx.x] = h[i];
 }
 
-// Exclusive scan over counts in key-major order; tier t starts at key t*kBins.
-__global__ void __launch_bounds__(1024) key_scan_kernel(long long *counts, int nb, TcCtl *ctl)
+// Exclusive scan over the (key, block) counts, in place, in the list order
+// tier -> superblock -> cost bin -> block: with nsb superblocks of
+// consecutive vertex blocks, each tier drains the vertex ranges in turn,
+// longest-first inside each (nsb = 1: longest-first over the whole tier).
+// A batch of graphs uses nsb > 1 so that graphs complete progressively.
+// Tier t starts at the first cell of its range.
+__device__ __forceinline__ long long scan_cell(long long i, int nb, int nsb)
+{
+    const long long per_tier = (long long)kBins * nb;
+    const long long t = i / per_tier, r = i % per_tier;
+    const int B = (nb + nsb - 1) / nsb;  // blocks per superblock
+    const int nsb_used = (nb + B - 1) / B;
+    long long sb = r / ((long long)kBins * B);
+    sb = sb < nsb_used - 1 ? sb : nsb_used - 1;
+    const long long rr = r - sb * kBins * B;
+    const long long b0 = sb * B;
+    const long long sz = (long long)min(nb, (int)(b0 + B)) - b0;
+    const long long bin = rr / sz, blk = b0 + rr % sz;
+    return (t * kBins + bin) * nb + blk;
+}
+
+__global__ void __launch_bounds__(1024) key_scan_kernel(long long *counts, int nb, int nsb,
+                                                        TcCtl *ctl)
 {
     __shared__ long long part[1024];
     __shared__ long long base;
@@ -215,9 +299,11 @@ __global__ void __launch_bounds__(1024) key_scan_kernel(long long *counts, int n
         base = 0;
     __syncthreads();
     const long long n = (long long)kKeys * nb;
+    const long long per_tier = (long long)kBins * nb;
     for (long long c0 = 0; c0 < n; c0 += blockDim.x) {
         const long long i = c0 + threadIdx.x;
-        const long long x = i < n ? counts[i] : 0;
+        const long long cell = i < n ? scan_cell(i, nb, nsb) : 0;
+        const long long x = i < n ? counts[cell] : 0;
         part[threadIdx.x] = x;
         __syncthreads();
         for (int off = 1; off < (int)blockDim.x; off <<= 1) {
@@ -229,10 +315,9 @@ __global__ void __launch_bounds__(1024) key_scan_kernel(long long *counts, int n
         const long long incl = part[threadIdx.x];
         if (i < n) {
             const long long off = base + incl - x;
-            counts[i] = off;
-            const long long key = i / nb, blk = i % nb;
-            if (blk == 0 && key % kBins == 0)
-                ctl->list_begin[key / kBins] = off;
+            counts[cell] = off;
+            if (i % per_tier == 0)
+                ctl->list_begin[i / per_tier] = off;
         }
         __syncthreads();
         if (threadIdx.x == blockDim.x - 1)
@@ -296,7 +381,7 @@ static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s)
     const long long V = ctx->g.V;
     if (V == 0)
         return TRUST_OK;
-    const int blocks = (int)std::min<long long>((V + 3) / 4, (long long)ctx->num_cus * 32);
+    const int blocks = (int)std::min<long long>((V + 255) / 256, (long long)ctx->num_cus * 32);
     hipLaunchKernelGGL(features_kernel<OffT>, dim3(blocks), dim3(256), 0, s,
                        (const OffT *)ctx->g.beg.p, (const int32_t *)ctx->g.adj.p, V, feat);
     HIPCHK(hipGetLastError());
@@ -351,7 +436,9 @@ static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchP
                        (unsigned short *)ctx->tiers.p);
     hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, s,
                        (const unsigned short *)ctx->tiers.p, V, per_block, counts, nb);
-    hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, s, counts, nb, ctl);
+    // a batch drains its graphs in 32 vertex ranges (progressive completion)
+    const int nsb = ctx->batch_ngraphs > 1 ? 32 : 1;
+    hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, s, counts, nb, nsb, ctl);
     hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, s,
                        (const unsigned short *)ctx->tiers.p, V, per_block, (const long long *)counts,
                        nb, (int32_t *)ctx->list.p);

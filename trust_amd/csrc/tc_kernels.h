@@ -29,12 +29,18 @@ constexpr int kModeBatch = 3;
 
 // Device control block: queue cursors, list bounds and results.  Cleared
 // by a fill kernel before a run.
+// The wave queue is split over kQueueShards counters, each on its own
+// 128-byte line: one address sustains only about 1e8 atomics/s, which
+// bounds graphs of many small vertices.
+constexpr int kQueueShards = 8;
+constexpr int kShardStride = 32;  // unsigned ints per counter line
 struct TcCtl {
     unsigned long long count;       // triangles
     unsigned long long probes;      // hash lookups (W)
     unsigned int next[8];           // queue cursors, one per tier
     long long list_begin[8];        // tier t occupies list[list_begin[t], list_begin[t+1])
     unsigned long long pad[4];
+    unsigned int shard[kQueueShards * kShardStride];  // wave-queue cursors
 };
 
 // Three CSR views: the hash source (N+(u) that is tabled), the walk (the v

@@ -699,15 +699,25 @@ tc_count_kernel(TcArgs<OffT> a)
         const long long lb = a.ctl->list_begin[a.wave_tier];
         const long long le = a.ctl->list_begin[a.wave_tier + 1];
         int *wtab = lds + wave * a.wave_slots;
-        for (;;) {
+        // Shard s hands out the rank's chunks j with j % kQueueShards == s
+        // (chunk c = j * nranks + rank, the reference's interleave), so the
+        // list order is kept across shards.  A wave starts on shard
+        // blockIdx % kQueueShards and moves on when a shard runs dry; after
+        // finding every shard dry once it is done.
+        int shard = (int)(blockIdx.x % kQueueShards);
+        int dry = 0;
+        while (dry < kQueueShards) {
             unsigned int k = 0;
             if (lane == 0)
-                k = atomicAdd(&a.ctl->next[a.wave_tier], 1u);
+                k = atomicAdd(&a.ctl->shard[shard * kShardStride], 1u);
             k = __builtin_amdgcn_readfirstlane(k);
-            const long long c = (long long)k * a.nranks + a.rank;
-            const long long first = lb + c * a.chunk_wave;
-            if (first >= le)
-                break;
+            const long long j = (long long)k * kQueueShards + shard;
+            const long long first = lb + (j * a.nranks + a.rank) * a.chunk_wave;
+            if (first >= le) {
+                dry++;
+                shard = shard + 1 == kQueueShards ? 0 : shard + 1;
+                continue;
+            }
             const long long last = min(first + (long long)a.chunk_wave, le);
             for (long long i = first; i < last; i++)
                 wave_vertex<OffT, kMode>(a, a.list[i], wtab, mark, acc, probes, ga);
