@@ -29,6 +29,8 @@ extern "C" {
 /* ---- schedules and flags ---------------------------------------------- */
 #define TRUST_SCHED_DEGREE 0  /* plain degree-threshold split (reference USE_CTA) */
 #define TRUST_SCHED_LEARNED 1 /* MLP cost model: tiers + longest-first queue */
+#define TRUST_SCHED_AUTO 2    /* LEARNED unless the degree split counted this graph faster
+                                 when the model was trained on it (the split as judge) */
 
 #define TRUST_FLAG_FORCE_SPILL 1u  /* every hashed vertex uses an HBM table */
 #define TRUST_FLAG_OFFSETS64 2u    /* keep int64 offsets on the device even if E < 2^31 */
@@ -106,7 +108,9 @@ typedef struct trust_ctx trust_ctx;
 const char *trust_last_error(void);
 const char *trust_version(void);
 /* Defaults: threads 0, blocks 0, chunk 0, rank 0, total_process 1,
- * schedule DEGREE, hub_threshold 100, wave_slots 0, lds_budget 0. */
+ * schedule DEGREE, hub_threshold 100, wave_slots 0, lds_budget 0.  With
+ * schedule LEARNED or AUTO and no cost model loaded, the degree split runs;
+ * trust_result.schedule_used says which schedule ran. */
 void trust_params_init(trust_params *p);
 /* Number of HIP devices (0 when none); negative on runtime failure. */
 int trust_device_count(void);
@@ -187,6 +191,17 @@ int trust_ctx_features(trust_ctx *ctx, float *feat);
 int trust_ctx_train_cost_model(trust_ctx *ctx, const trust_params *p, int epochs,
                                int64_t max_samples, uint64_t seed, double *loss, double *seconds);
 int trust_ctx_set_cost_model(trust_ctx *ctx, int hidden, const float *params);
+/*
+ * The degree split's verdict on the trained model for the resident graph:
+ * trust_ctx_train_cost_model ends with two learned and two degree-split
+ * counts of the graph at the caller's launch parameters (their device time is
+ * included in its *seconds).  *decision = TRUST_SCHED_LEARNED or
+ * TRUST_SCHED_DEGREE: LEARNED only when its best of two was more than 1% faster; or -1 when no
+ * judgement exists (no training on this graph, or a model set by
+ * trust_ctx_set_cost_model); TRUST_SCHED_AUTO follows it.  Times nullable.
+ */
+int trust_ctx_schedule_judgement(trust_ctx *ctx, int *decision, double *learned_seconds,
+                                 double *degree_seconds);
 int trust_ctx_get_cost_model(trust_ctx *ctx, int *hidden, float *params);
 /* Predicted single-wave cycles per vertex (0 for vertices that are not counted). */
 int trust_ctx_predict_costs(trust_ctx *ctx, float *cost);

@@ -144,6 +144,31 @@ def test_learned_schedule_counts_and_predicts(gpu, rmat16):
             assert r2.triangles == want and r2.schedule_used == T.SCHED_LEARNED
 
 
+def test_judge_and_auto_schedule(gpu, rmat16):
+    beg, adj, want = rmat16
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        assert c.schedule_judgement()[0] == -1
+        r = c.count(schedule=T.SCHED_AUTO)  # no model: the degree split
+        assert r.triangles == want and r.schedule_used == T.SCHED_DEGREE
+        c.train_cost_model(epochs=5, max_samples=20000, seed=1)
+        decision, ls, ds = c.schedule_judgement()
+        assert decision in (T.SCHED_LEARNED, T.SCHED_DEGREE) and ls > 0 and ds > 0
+        assert decision == (T.SCHED_LEARNED if ls < 0.99 * ds else T.SCHED_DEGREE)
+        r = c.count(schedule=T.SCHED_AUTO)
+        assert r.triangles == want and r.schedule_used == decision
+        # forced learned still runs the model
+        assert c.count(schedule=T.SCHED_LEARNED).schedule_used == T.SCHED_LEARNED
+        # a model set by hand has no verdict: AUTO runs it
+        c.set_cost_model(c.get_cost_model())
+        assert c.schedule_judgement()[0] == -1
+        assert c.count(schedule=T.SCHED_AUTO).schedule_used == T.SCHED_LEARNED
+        # a new graph drops the verdict but keeps the model
+        c.load(beg, adj)
+        assert c.schedule_judgement()[0] == -1
+        assert c.count(schedule=T.SCHED_AUTO).schedule_used == T.SCHED_LEARNED
+
+
 def test_learned_gain_reported(gpu):
     # what the trained network changes against the degree split on a
     # mid-size R-MAT: both exact; times printed for the record

@@ -25,7 +25,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 
 TRUST_OK = 0
 ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2, -3, -4, -5, -6, -7)
-SCHED_DEGREE, SCHED_LEARNED = 0, 1
+SCHED_DEGREE, SCHED_LEARNED, SCHED_AUTO = 0, 1, 2
 FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
 FLAG_HIP_GRAPH = 64
 # launch parameters the autotuner searches
@@ -110,6 +110,8 @@ SIGNATURES = {
                                              C.c_uint64, C.POINTER(C.c_double),
                                              C.POINTER(C.c_double)]),
     "trust_ctx_set_cost_model": (C.c_int, [C.c_void_p, C.c_int, _PF]),
+    "trust_ctx_schedule_judgement": (C.c_int, [C.c_void_p, C.POINTER(C.c_int),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "trust_ctx_get_cost_model": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _PF]),
     "trust_ctx_predict_costs": (C.c_int, [C.c_void_p, _PF]),
     "trust_ctx_autotune": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, C.c_double,
@@ -465,6 +467,13 @@ class Context:
     def set_cost_model(self, params):
         params = np.ascontiguousarray(params, np.float32)
         _check(lib().trust_ctx_set_cost_model(self._h, MLP_HIDDEN, params.ctypes.data_as(_PF)))
+
+    def schedule_judgement(self):
+        """(decision, learned seconds, degree seconds) of the last training on this
+        graph; decision is SCHED_LEARNED, SCHED_DEGREE or -1 (not judged)."""
+        d, ls, ds = C.c_int(0), C.c_double(0), C.c_double(0)
+        _check(lib().trust_ctx_schedule_judgement(self._h, C.byref(d), C.byref(ls), C.byref(ds)))
+        return d.value, ls.value, ds.value
 
     def get_cost_model(self) -> np.ndarray:
         h = C.c_int(0)

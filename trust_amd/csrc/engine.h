@@ -87,6 +87,10 @@ struct Ctx {
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     bool mlp_ready = false;
+    // the degree split as judge: which schedule counted the resident graph
+    // faster when the model was trained on it (-1: not judged)
+    int judge = -1;
+    double judge_learned_s = 0, judge_degree_s = 0;
     // batch of graphs: vbase[ngraphs + 1]; out = counts[ng], done[ng], t0
     DevBuf batch_vbase, batch_page, batch_out;
     int64_t batch_ngraphs = 0;

@@ -313,7 +313,8 @@ int check_params(const trust_params &p)
         return fail(TRUST_ERR_ARG, "hub_threshold %d < 0", p.hub_threshold);
     if (p.chunk < 0 || p.blocks < 0 || p.threads < 0)
         return fail(TRUST_ERR_ARG, "negative launch parameter");
-    if (p.schedule != TRUST_SCHED_DEGREE && p.schedule != TRUST_SCHED_LEARNED)
+    if (p.schedule != TRUST_SCHED_DEGREE && p.schedule != TRUST_SCHED_LEARNED &&
+        p.schedule != TRUST_SCHED_AUTO)
         return fail(TRUST_ERR_ARG, "unknown schedule %d", p.schedule);
     const uint32_t known = TRUST_FLAG_FORCE_SPILL | TRUST_FLAG_OFFSETS64 | TRUST_FLAG_NO_VALIDATE |
                            TRUST_FLAG_WAVE_ONLY | TRUST_FLAG_LINEAR_PROBING | TRUST_FLAG_HIP_GRAPH;
@@ -356,8 +357,10 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
         if (rc)
             return rc;
     }
-    // learned schedule when a cost model is loaded, else the degree split
-    R.learned = p.schedule == TRUST_SCHED_LEARNED && ctx->mlp_ready;
+    // learned schedule when a cost model is loaded (AUTO: unless the judge
+    // found the degree split faster on this graph), else the degree split
+    R.learned = ctx->mlp_ready && (p.schedule == TRUST_SCHED_LEARNED ||
+                                   (p.schedule == TRUST_SCHED_AUTO && ctx->judge != TRUST_SCHED_DEGREE));
     return TRUST_OK;
 }
 
@@ -512,6 +515,7 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
     // state derived from the previous graph
     drop_captured(ctx);
     ctx->batch_ngraphs = 0;
+    ctx->judge = -1;  // the judgement belongs to the graph; the model stays
     auto t0 = std::chrono::steady_clock::now();
     int rc = upload_csr(ctx, ctx->g, beg, adj, V, E, (flags & TRUST_FLAG_OFFSETS64) != 0,
                         ctx->stream);

@@ -534,9 +534,29 @@ int train_cost_model(Ctx *ctx, const trust_params &p_in, int epochs, long long m
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[2]));
-    if (seconds_out)
-        *seconds_out = t0 + ms * 1e-3;
     ctx->mlp_ready = true;
+    // the degree split judges the trained model: one learned and one degree
+    // count of this graph (best of two each) with the caller's launch
+    double judge_s = 0;
+    for (int sched : {TRUST_SCHED_LEARNED, TRUST_SCHED_DEGREE}) {
+        trust_params q = p_in;
+        q.schedule = sched;
+        double best = 1e300;
+        for (int i = 0; i < 2; i++) {
+            trust_result rj;
+            if ((rc = count_resident(ctx, q, kModeCount, nullptr, nullptr, &rj)))
+                return rc;
+            best = std::min(best, rj.seconds);
+            judge_s += rj.seconds;
+        }
+        (sched == TRUST_SCHED_LEARNED ? ctx->judge_learned_s : ctx->judge_degree_s) = best;
+    }
+    // the learned schedule must win by more than run-to-run noise (1%): a
+    // tie goes to the degree split, which has no plan to pay for
+    ctx->judge = ctx->judge_learned_s < 0.99 * ctx->judge_degree_s ? TRUST_SCHED_LEARNED
+                                                                    : TRUST_SCHED_DEGREE;
+    if (seconds_out)
+        *seconds_out = t0 + ms * 1e-3 + judge_s;
     return TRUST_OK;
 }
 
@@ -586,6 +606,22 @@ extern "C" int trust_ctx_set_cost_model(trust_ctx *c, int hidden, const float *p
         return rc;
     HIPCHK(hipMemcpy(ctx->mlp_params.p, params, sizeof(float) * P, hipMemcpyHostToDevice));
     ctx->mlp_ready = true;
+    ctx->judge = -1;
+    return TRUST_OK;
+}
+
+extern "C" int trust_ctx_schedule_judgement(trust_ctx *c, int *decision, double *learned_seconds,
+                                            double *degree_seconds)
+{
+    clear_error();
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    if (!ctx || !decision)
+        return fail(TRUST_ERR_ARG, "null argument");
+    *decision = ctx->judge;
+    if (learned_seconds)
+        *learned_seconds = ctx->judge >= 0 ? ctx->judge_learned_s : 0.0;
+    if (degree_seconds)
+        *degree_seconds = ctx->judge >= 0 ? ctx->judge_degree_s : 0.0;
     return TRUST_OK;
 }
 
