@@ -195,10 +195,11 @@ int trust_ctx_set_cost_model(trust_ctx *ctx, int hidden, const float *params);
  * The degree split's verdict on the trained model for the resident graph:
  * trust_ctx_train_cost_model ends with two learned and two degree-split
  * counts of the graph at the caller's launch parameters (their device time is
- * included in its *seconds).  *decision = TRUST_SCHED_LEARNED or
- * TRUST_SCHED_DEGREE: LEARNED only when its best of two was more than 1% faster; or -1 when no
- * judgement exists (no training on this graph, or a model set by
- * trust_ctx_set_cost_model); TRUST_SCHED_AUTO follows it.  Times nullable.
+ * included in its *seconds).  *decision = TRUST_SCHED_LEARNED when the
+ * learned count's best of two was more than 1% faster, else
+ * TRUST_SCHED_DEGREE; -1 when no judgement exists (no training on this graph,
+ * or a model set by trust_ctx_set_cost_model).  TRUST_SCHED_AUTO follows it.
+ * Times nullable.
  */
 int trust_ctx_schedule_judgement(trust_ctx *ctx, int *decision, double *learned_seconds,
                                  double *degree_seconds);
