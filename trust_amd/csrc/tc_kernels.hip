@@ -41,6 +41,7 @@
 namespace trust {
 
 constexpr int kEmpty = -1;
+constexpr int kNoProbe = -2;  // a window position past the end: matches no slot
 
 // ---------------------------------------------------------------- wave ops
 __device__ __forceinline__ void wave_sync()
@@ -169,17 +170,22 @@ __device__ __forceinline__ int table_lg(int d, int cap_lg)
 
 // ---- bucketed two-choice cuckoo table (the lookup path) ----
 // 2^lg int slots = 2^(lg-1) buckets of two slots.  A key lives in bucket
-// b1 = key & mask or b2 = Fibonacci hash of key, so a lookup is exactly two
-// 8-byte reads and branch-free compares: no probe chain whose longest lane
-// would stall the whole wave.  Load factor <= 1/2, far below the ~0.89
-// limit of 2-way, 2-slot cuckoo hashing.
+// b1 = key & mask or b2 = a multiplicative hash of key, so a lookup is
+// exactly two 8-byte reads and branch-free compares: no probe chain whose
+// longest lane would stall the whole wave.  Load factor <= 1/2, far below
+// the ~0.89 limit of 2-way, 2-slot cuckoo hashing.  b2 multiplies the low
+// 24 bits (a full-rate v_mul_u32_u24; the 32-bit multiply is quarter rate
+// and the walk is instruction-bound) and takes the top bits of the low word;
+// keys equal in their low 24 bits share both buckets, which a bucket pair's
+// four slots absorb, and a failed insertion falls back to linear probing.
 __device__ __forceinline__ unsigned int bucket1(int key, int lgb)
 {
     return (unsigned int)key & ((1u << lgb) - 1u);
 }
 __device__ __forceinline__ unsigned int bucket2(int key, int lgb)
 {
-    return ((unsigned int)key * 0x9E3779B1u) >> (32 - lgb);
+    // __umul24 returns int: shift the unsigned value (a logical shift)
+    return (unsigned int)__umul24((unsigned int)key, 0x9E3779u) >> (32 - lgb);
 }
 
 template <bool kGlobal>
@@ -223,26 +229,32 @@ __device__ __forceinline__ bool cuckoo_insert(int *tab, int lgb, int key)
     return true;
 }
 
-// Membership count of up to kUnroll keys (key < 0: not a probe).  Both
-// buckets of every key are read at once: reading the second bucket only
-// when the first is full (sound, as slots are never vacated) saves LDS reads
-// but serialises two LDS round trips, and measured 12% slower at R-MAT 24.
-template <bool kGlobal>
-__device__ __forceinline__ unsigned int cuckoo_count(const int *tab, int lg, const int (&w)[kUnroll])
+// Membership count of kUnroll keys (kNoProbe matches no slot), summed over
+// the wave: the per-key hit masks are counted with scalar popcounts, which
+// keeps vector instructions (the walk's bound) for the lookup itself.  All
+// 64 lanes must call it.  Both buckets of every key are read at once:
+// reading the second bucket only when the first is full (sound, as slots
+// are never vacated) saves LDS reads but serialises two LDS round trips, and
+// measured 12% slower at R-MAT 24.
+template <bool kGlobal, int N>
+__device__ __forceinline__ unsigned int cuckoo_count(const int *tab, int lg, const int (&w)[N])
 {
     const int lgb = lg - 1;
-    int2 x1[kUnroll], x2[kUnroll];
+    int2 x1[N], x2[N];
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) {
+    for (int k = 0; k < N; k++) {
         x1[k] = bucket_load<kGlobal>(tab, bucket1(w[k], lgb));
         x2[k] = bucket_load<kGlobal>(tab, bucket2(w[k], lgb));
     }
     unsigned int hits = 0;
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) {
-        const bool in = (x1[k].x == w[k]) | (x1[k].y == w[k]) | (x2[k].x == w[k]) |
-                        (x2[k].y == w[k]);
-        hits += (unsigned int)(in & (w[k] >= 0));
+    for (int k = 0; k < N; k++) {
+        // each compare's lane mask feeds a scalar OR and popcount directly
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(x1[k].x == w[k]) |
+                                     __builtin_amdgcn_ballot_w64(x1[k].y == w[k]) |
+                                     __builtin_amdgcn_ballot_w64(x2[k].x == w[k]) |
+                                     __builtin_amdgcn_ballot_w64(x2[k].y == w[k]);
+        hits += (unsigned int)__popcll(m);
     }
     return hits;
 }
@@ -284,22 +296,23 @@ __device__ __forceinline__ void table_insert(int *tab, int lg, int key)
     }
 }
 
-// Membership of up to kUnroll keys (key < 0: skip).  First slots of all
-// keys are read together, then each key finishes its probe sequence.
-template <bool kGlobal>
-__device__ __forceinline__ unsigned int linear_count(const int *tab, int lg, const int (&w)[kUnroll])
+// Membership of up to kUnroll keys (key < 0: skip), summed over the wave.
+// First slots of all keys are read together, then each key finishes its
+// probe sequence.  All 64 lanes must call it.
+template <bool kGlobal, int N>
+__device__ __forceinline__ unsigned int linear_count(const int *tab, int lg, const int (&w)[N])
 {
     const unsigned int mask = (1u << lg) - 1u;
-    unsigned int h[kUnroll];
-    int s[kUnroll];
+    unsigned int h[N];
+    int s[N];
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) {
+    for (int k = 0; k < N; k++) {
         h[k] = hash_slot(w[k], lg);
         s[k] = w[k] >= 0 ? slot_load<kGlobal>(tab + h[k]) : kEmpty;
     }
     unsigned int hits = 0;
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) {
+    for (int k = 0; k < N; k++) {
         if (w[k] < 0)
             continue;
         for (;;) {
@@ -313,7 +326,7 @@ __device__ __forceinline__ unsigned int linear_count(const int *tab, int lg, con
             s[k] = slot_load<kGlobal>(tab + h[k]);
         }
     }
-    return hits;
+    return (unsigned int)wave_sum((unsigned long long)hits);
 }
 
 // ---------------------------------------------------------- probe walk
@@ -346,78 +359,98 @@ __device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT 
 }
 
 // Probes windows [wbeg, wend) (64 positions each) of a group's
-// concatenation against the table.  All 64 lanes must call it.  `mark` is
-// this wave's 64-int LDS scratch: all-zero on entry and on exit.
+// concatenation against the table and returns the hits of the whole wave
+// (wave-uniform; gather mode: the wave's checksum).  All 64 lanes must call
+// it.  `mark` is this wave's 64-int LDS scratch: all-zero on entry and exit.
 template <typename OffT, bool kGlobal, int kMode = kModeCount>
-__device__ __forceinline__ unsigned int probe_windows(const Group<OffT> &g, int wbeg, int wend,
+__device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g, int wbeg, int wend,
                                                       const int32_t *__restrict__ adjP,
                                                       const int *tab, int lg, bool linear,
                                                       int *mark)
 {
     const int lane = lane_id();
     const int bend = min(wend * kWave, g.total);
-    int b = wbeg * kWave;
+    const int b = wbeg * kWave;
     if (b >= bend)
         return 0;
     // segment holding position b = number of lists ending at or before b
     int carry = (int)__popcll(__ballot(g.incl <= b));
-    // Issues the loads of the kUnroll windows starting at position `at`
-    // (window positions past bend load nothing and yield -1).
-    auto fetch = [&](int at, int (&w)[kUnroll]) {
-        OffT bj[kUnroll];
-#pragma unroll
-        for (int k = 0; k < kUnroll; k++) {
-            const int bk = at + k * kWave;
-            bj[k] = 0;
-            if (bk >= bend)
-                continue;
-            if (readlane(g.incl, carry) >= bk + kWave) {
-                bj[k] = readlane(g.base, carry);  // the window lies inside one list
-            } else {
-                if (g.deg > 0 && g.excl >= bk && g.excl < bk + kWave)
-                    mark[g.excl - bk] = lane;
-                wave_sync();
-                const int m = mark[lane];
-                mark[lane] = 0;
-                wave_sync();
-                const int j = max(wave_scan_max(m), carry);
-                carry = readlane(j, 63);
-                bj[k] = shfl_idx(g.base, j);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kUnroll; k++) {
-            const int t = at + k * kWave + lane;
-            w[k] = t < bend ? adjP[bj[k] + (OffT)t] : -1;
-        }
+    // Base of the window starting at position bk (< bend): adjP[base + t] is
+    // position t.  A window inside one list reads its base with a readlane;
+    // otherwise each position's list comes from the mark array and a max-scan.
+    auto window_base = [&](int bk) -> OffT {
+        if (readlane(g.incl, carry) >= bk + kWave)
+            return readlane(g.base, carry);
+        if (g.deg > 0 && g.excl >= bk && g.excl < bk + kWave)
+            mark[g.excl - bk] = lane;
+        wave_sync();
+        const int m = mark[lane];
+        mark[lane] = 0;
+        wave_sync();
+        const int j = max(wave_scan_max(m), carry);
+        carry = readlane(j, 63);
+        return shfl_idx(g.base, j);
     };
-    // Software pipeline: the next kUnroll windows' loads are in flight while
-    // the current ones are looked up.
-    unsigned int hits = 0;
-    int wcur[kUnroll];
-    fetch(b, wcur);
-    for (;;) {
-        const int bn = b + kWave * kUnroll;
-        const bool more = bn < bend;
-        int wnext[kUnroll];
-        if (more)
-            fetch(bn, wnext);
+    unsigned long long total = 0;  // wave-uniform hits
+    unsigned int checksum = 0;     // gather mode, per lane
+    auto lookup = [&](const auto &w) {
+        constexpr int N = (int)(sizeof(w) / sizeof(int));
         if constexpr (kMode == kModeGather) {
 #pragma unroll
-            for (int k = 0; k < kUnroll; k++)
-                hits += wcur[k] >= 0 ? (unsigned int)wcur[k] : 0u;
+            for (int k = 0; k < N; k++)
+                checksum += w[k] >= 0 ? (unsigned int)w[k] : 0u;
         } else {
-            hits += linear ? linear_count<kGlobal>(tab, lg, wcur)
-                           : cuckoo_count<kGlobal>(tab, lg, wcur);
+            total += linear ? linear_count<kGlobal, N>(tab, lg, w)
+                            : cuckoo_count<kGlobal, N>(tab, lg, w);
         }
-        if (!more)
-            break;
+    };
+    if (bend - b <= kWave) {
+        // a single window (most small vertices): one load, one lookup
+        const OffT bj = window_base(b);
+        const int t = b + lane;
+        const int w1[1] = {t < bend ? adjP[bj + (OffT)t] : kNoProbe};
+        lookup(w1);
+    } else {
+        // Issues the loads of the kUnroll windows starting at position `at`.
+        // Every lane loads (positions past bend read adjP[0], which exists
+        // since bend > 0) and nothing reads the results here: with no branch
+        // around the loads the compiler knows exactly kUnroll loads are
+        // younger than the previous batch and waits for that batch alone.
+        auto fetch = [&](int at, int (&w)[kUnroll]) {
+            OffT bj[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++)
+                bj[k] = at + k * kWave < bend ? window_base(at + k * kWave) : (OffT)0;
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++) {
+                const int t = at + k * kWave + lane;
+                w[k] = adjP[t < bend ? bj[k] + (OffT)t : (OffT)0];
+            }
+        };
+        // Software pipeline: the next kUnroll windows' loads are in flight
+        // while the current ones are looked up.  Every batch inside the loop
+        // lies before bend; the last batch, looked up after the loop, may run
+        // past it and its positions there become kNoProbe.
+        int wcur[kUnroll];
+        fetch(b, wcur);
+        int last = b;
+        for (int bn = b + kWave * kUnroll; bn < bend; bn += kWave * kUnroll) {
+            int wnext[kUnroll];
+            fetch(bn, wnext);
+            lookup(wcur);
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++)
+                wcur[k] = wnext[k];
+            last = bn;
+        }
 #pragma unroll
         for (int k = 0; k < kUnroll; k++)
-            wcur[k] = wnext[k];
-        b = bn;
+            wcur[k] = last + k * kWave + lane < bend ? wcur[k] : kNoProbe;
+        lookup(wcur);
     }
-    return hits;
+    if constexpr (kMode == kModeGather)
+        total += wave_sum((unsigned long long)checksum);
+    return total;
 }
 
 // ------------------------------------------------------- per-vertex work
@@ -483,7 +516,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
         __threadfence();
     __syncthreads();
     const int nchunks = (dW + kWave - 1) / kWave;
-    unsigned int hits = 0;
+    unsigned long long hits = 0;  // wave-uniform
     unsigned long long wave_probes = 0;
     if (nchunks <= kMaxChunks) {
         // window count of every 64-neighbour chunk, then an exclusive prefix
@@ -565,7 +598,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
                                                  lg, linear, mark);
         }
     }
-    acc += hits;
+    acc += lane == 0 ? hits : 0ull;  // the wave's hits count once
     probes += wave_probes;
     __syncthreads();
     if constexpr (kInstr) {
@@ -616,7 +649,7 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
             wave_sync();
         }
     }
-    unsigned int hits = 0;
+    unsigned long long hits = 0;  // wave-uniform
     unsigned long long wave_probes = 0;
     for (int c = 0; c < dW; c += kWave) {
         const bool valid = c + lane < dW;
@@ -626,7 +659,7 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
                                            linear, mark);
     }
     wave_sync();
-    acc += hits;
+    acc += lane == 0 ? hits : 0ull;  // the wave's hits count once
     probes += wave_probes;
     if constexpr (kInstr) {
         if (lane == 0) {
