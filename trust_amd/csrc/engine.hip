@@ -99,11 +99,12 @@ static long long fair_table_words(long long lds_bytes, int threads)
     return share > 0 ? 1ll << floor_lg(share) : 0;
 }
 
-static int shape_occupancy(Ctx *ctx, LaunchShape &s, bool off64, int blocks_req, int mode)
+static int shape_occupancy(Ctx *ctx, LaunchShape &s, bool off64, int blocks_req, int mode,
+                           bool big)
 {
     int per_cu = 0;
-    hipError_t e = off64 ? count_occupancy<int64_t>(s.threads, s.lds_bytes, mode, &per_cu)
-                         : count_occupancy<int32_t>(s.threads, s.lds_bytes, mode, &per_cu);
+    hipError_t e = off64 ? count_occupancy<int64_t>(s.threads, s.lds_bytes, mode, big, &per_cu)
+                         : count_occupancy<int32_t>(s.threads, s.lds_bytes, mode, big, &per_cu);
     if (e != hipSuccess)
         return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (per_cu < 1)
@@ -142,7 +143,7 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
     m.table_words = (int)std::max(wave_words, main_words);
     m.group_lg = floor_lg(m.table_words);
     m.lds_bytes = (size_t)(m.table_words + lds_extra_words(m.threads)) * 4;
-    int rc = shape_occupancy(ctx, m, off64, p.blocks, mode);
+    int rc = shape_occupancy(ctx, m, off64, p.blocks, mode, false);
     if (rc)
         return rc;
     if (p.chunk > 0) {
@@ -179,7 +180,7 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
             b.group_lg = 6;
             b.lds_bytes = (size_t)(64 + lds_extra_words(64)) * 4;
         }
-        rc = shape_occupancy(ctx, b, off64, 0, mode);
+        rc = shape_occupancy(ctx, b, off64, 0, mode, true);
         if (rc)
             return rc;
         if (need_lg > b.group_lg || force_spill) {
@@ -283,13 +284,13 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         a.spill_tier = L.spill_groups > 0 ? kTierSpill : -1;
         a.group_tier = kTierBig;
         a.wave_tier = -1;
-        HIPCHK(launch_count<OffT>(a, L.big.blocks, L.big.threads, L.big.lds_bytes, mode, s));
+        HIPCHK(launch_count<OffT>(a, L.big.blocks, L.big.threads, L.big.lds_bytes, mode, true, s));
     }
     fill_args(a, ctx, views, p, L, L.main, ctl_dev, instr_cycles, instr_probes);
     a.spill_tier = -1;
     a.group_tier = kTierHub;
     a.wave_tier = kTierWave;
-    HIPCHK(launch_count<OffT>(a, L.main.blocks, L.main.threads, L.main.lds_bytes, mode, s));
+    HIPCHK(launch_count<OffT>(a, L.main.blocks, L.main.threads, L.main.lds_bytes, mode, false, s));
     if (ev_count)
         HIPCHK(hipEventRecord(ev_count, s));
     return TRUST_OK;

@@ -111,10 +111,11 @@ struct PlanArgs {
 template <typename OffT>
 hipError_t launch_plan(const PlanArgs<OffT> &p, int nblocks, hipStream_t stream);
 template <typename OffT>
+// big: the large-LDS launch (spill + big tiers), else the main launch.
 hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
-                        int mode, hipStream_t stream);
+                        int mode, bool big, hipStream_t stream);
 template <typename OffT>
-hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, int *blocks_per_cu);
+hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, bool big, int *blocks_per_cu);
 hipError_t launch_copy(const int4 *src, int4 *dst, long long n, int blocks, hipStream_t stream);
 hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
                                  hipStream_t stream);

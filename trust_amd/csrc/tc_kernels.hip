@@ -698,8 +698,11 @@ __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int
 // Eight waves per SIMD: the compiler fits the kernel in 64 VGPRs (a few
 // spills outside the probe loop).  The walk is latency-bound, and the
 // eighth wave measured 15% faster at R-MAT 24 than seven waves without
-// spills (1.6x at 1024-thread workgroups).
-template <typename OffT, int kMode>
+// spills (1.6x at 1024-thread workgroups).  kBig: the large-LDS launch
+// (HBM-table spill tier + big tier); otherwise the main launch (hub + wave
+// tiers).  Each is compiled with only its own phases, so the registers of
+// one do not constrain the other.
+template <typename OffT, int kMode, bool kBig>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
 tc_count_kernel(TcArgs<OffT> a)
 {
@@ -716,7 +719,7 @@ tc_count_kernel(TcArgs<OffT> a)
     if (kMode == kModeBatch && threadIdx.x == 0)
         atomicMin(a.graph_t0, (unsigned long long)wall_clock64());
 
-    if (a.spill_tier >= 0 && (int)blockIdx.x < a.spill_groups) {
+    if (kBig && a.spill_tier >= 0 && (int)blockIdx.x < a.spill_groups) {
         int *tab = a.spill_mem + (long long)blockIdx.x * a.spill_slots;
         int cap_lg = 0;
         while ((2ll << cap_lg) <= a.spill_slots)
@@ -728,7 +731,7 @@ tc_count_kernel(TcArgs<OffT> a)
         group_phase<OffT, kMode, false>(a, a.group_tier, lds, a.group_lg, mark, misc, ctot, acc,
                                         probes, ga);
     __syncthreads();  // the wave tables reuse the group table's LDS
-    if (a.wave_tier >= 0) {
+    if (!kBig && a.wave_tier >= 0) {
         const long long lb = a.ctl->list_begin[a.wave_tier];
         const long long le = a.ctl->list_begin[a.wave_tier + 1];
         int *wtab = lds + wave * a.wave_slots;
@@ -939,39 +942,56 @@ hipError_t launch_plan(const PlanArgs<OffT> &p, int nblocks, hipStream_t stream)
     return hipGetLastError();
 }
 
-template <typename OffT>
-hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
-                        int mode, hipStream_t stream)
+template <typename OffT, bool kBig>
+static void launch_count_t(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
+                           int mode, hipStream_t stream)
 {
     if (mode == kModeInstr)
-        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeInstr>), dim3(blocks), dim3(threads),
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeInstr, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
     else if (mode == kModeGather)
-        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeGather>), dim3(blocks), dim3(threads),
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeGather, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
     else if (mode == kModeBatch)
-        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeBatch>), dim3(blocks), dim3(threads),
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeBatch, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
     else
-        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeCount>), dim3(blocks), dim3(threads),
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeCount, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
-    return hipGetLastError();
 }
 
 template <typename OffT>
-hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, int *blocks_per_cu)
+hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
+                        int mode, bool big, hipStream_t stream)
+{
+    if (big)
+        launch_count_t<OffT, true>(a, blocks, threads, lds_bytes, mode, stream);
+    else
+        launch_count_t<OffT, false>(a, blocks, threads, lds_bytes, mode, stream);
+    return hipGetLastError();
+}
+
+template <typename OffT, bool kBig>
+static hipError_t count_occupancy_t(int threads, size_t lds_bytes, int mode, int *blocks_per_cu)
 {
     if (mode == kModeInstr)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, tc_count_kernel<OffT, kModeInstr>, threads, lds_bytes);
+            blocks_per_cu, tc_count_kernel<OffT, kModeInstr, kBig>, threads, lds_bytes);
     if (mode == kModeGather)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, tc_count_kernel<OffT, kModeGather>, threads, lds_bytes);
+            blocks_per_cu, tc_count_kernel<OffT, kModeGather, kBig>, threads, lds_bytes);
     if (mode == kModeBatch)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, tc_count_kernel<OffT, kModeBatch>, threads, lds_bytes);
+            blocks_per_cu, tc_count_kernel<OffT, kModeBatch, kBig>, threads, lds_bytes);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, tc_count_kernel<OffT, kModeCount>, threads, lds_bytes);
+        blocks_per_cu, tc_count_kernel<OffT, kModeCount, kBig>, threads, lds_bytes);
+}
+
+template <typename OffT>
+hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, bool big, int *blocks_per_cu)
+{
+    return big ? count_occupancy_t<OffT, true>(threads, lds_bytes, mode, blocks_per_cu)
+               : count_occupancy_t<OffT, false>(threads, lds_bytes, mode, blocks_per_cu);
 }
 
 hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n, hipStream_t stream)
@@ -994,9 +1014,11 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream
 
 template hipError_t launch_plan<int32_t>(const PlanArgs<int32_t> &, int, hipStream_t);
 template hipError_t launch_plan<int64_t>(const PlanArgs<int64_t> &, int, hipStream_t);
-template hipError_t launch_count<int32_t>(const TcArgs<int32_t> &, int, int, size_t, int, hipStream_t);
-template hipError_t launch_count<int64_t>(const TcArgs<int64_t> &, int, int, size_t, int, hipStream_t);
-template hipError_t count_occupancy<int32_t>(int, size_t, int, int *);
-template hipError_t count_occupancy<int64_t>(int, size_t, int, int *);
+template hipError_t launch_count<int32_t>(const TcArgs<int32_t> &, int, int, size_t, int, bool,
+                                          hipStream_t);
+template hipError_t launch_count<int64_t>(const TcArgs<int64_t> &, int, int, size_t, int, bool,
+                                          hipStream_t);
+template hipError_t count_occupancy<int32_t>(int, size_t, int, bool, int *);
+template hipError_t count_occupancy<int64_t>(int, size_t, int, bool, int *);
 
 }  // namespace trust
