@@ -33,7 +33,7 @@ extern "C" {
                                  when the model was trained on it (the split as judge) */
 
 #define TRUST_FLAG_FORCE_SPILL 1u  /* every hashed vertex uses an HBM table */
-#define TRUST_FLAG_OFFSETS64 2u    /* keep int64 offsets on the device even if E < 2^31 */
+#define TRUST_FLAG_OFFSETS64 2u    /* keep int64 offsets on the device even if E < 2^30 */
 #define TRUST_FLAG_NO_VALIDATE 4u  /* skip the host-side CSR validation */
 #define TRUST_FLAG_WAVE_ONLY 8u    /* no workgroup tier: every vertex that fits a wave table uses one wave */
 #define TRUST_FLAG_LINEAR_PROBING 16u /* linear-probing tables instead of two-choice cuckoo buckets */

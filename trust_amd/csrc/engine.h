@@ -32,7 +32,7 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
-// A CSR resident in HBM.  Offsets are int32 when E < 2^31 (halves the
+// A CSR resident in HBM.  Offsets are int32 when E < 2^30 (halves the
 // offset traffic of the probe walk) unless int64 is requested.
 struct DevCsr {
     DevBuf beg, adj;

@@ -55,7 +55,7 @@ int upload_csr(Ctx *ctx, DevCsr &g, const int64_t *beg, const int32_t *adj, int6
 {
     g.V = V;
     g.E = E;
-    g.off64 = force64 || E >= (int64_t)INT32_MAX;
+    g.off64 = force64 || E >= kMaxEdges32;
     g.max_deg = host_max_degree(beg, V);
     int rc = g.adj.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1));
     if (rc)

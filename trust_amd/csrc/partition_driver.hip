@@ -76,7 +76,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
     bool off64 = (p.flags & TRUST_FLAG_OFFSETS64) != 0;
     int64_t maxd = 0, Vsum = 0, Esum = 0;
     for (int t = 0; t < n * n; t++) {
-        off64 = off64 || tiles[t].edges >= (int64_t)INT32_MAX;
+        off64 = off64 || tiles[t].edges >= kMaxEdges32;
         for (int64_t u = 0; u < tiles[t].vertices; u++)
             maxd = std::max(maxd, tiles[t].beg[u + 1] - tiles[t].beg[u]);
         Esum += tiles[t].edges;
@@ -337,7 +337,7 @@ extern "C" int trust_triangle_count_partition(int rank, const char *folder, int 
     bool off64 = false;
     int64_t maxd = 0;
     for (int q = 0; q < 3; q++)
-        off64 = off64 || t[q].edges >= (int64_t)INT32_MAX;
+        off64 = off64 || t[q].edges >= kMaxEdges32;
     for (int64_t u = 0; u < t[0].vertices; u++)
         maxd = std::max(maxd, t[0].beg[u + 1] - t[0].beg[u]);
     DevCsr d[3];

@@ -18,7 +18,10 @@ constexpr int kWave = 64;
 constexpr int kMinTableLg = 6;       // smallest table: 64 slots
 constexpr int kPlanPerBlock = 4096;  // vertices per planning workgroup
 constexpr int kMaxChunks = 256;      // 64-neighbour chunks balanced per group vertex
-constexpr int kUnroll = 4;           // probe windows in flight per wave
+constexpr int kUnroll = 4;
+// int32 offsets (and 32-bit byte offsets into adj) are used when a CSR holds
+// fewer than kMaxEdges32 ids; larger ones keep int64 offsets.
+constexpr long long kMaxEdges32 = 1ll << 30;           // probe windows in flight per wave
 
 // Kernel modes: count triangles, count + per-vertex instrumentation, or
 // gather only (the same loads, every hash lookup replaced by a checksum of

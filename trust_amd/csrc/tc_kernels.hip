@@ -411,37 +411,66 @@ __device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g
         const int w1[1] = {t < bend ? adjP[bj + (OffT)t] : kNoProbe};
         lookup(w1);
     } else {
-        // Issues the loads of the kUnroll windows starting at position `at`.
-        // Every lane loads (positions past bend read adjP[0], which exists
-        // since bend > 0) and nothing reads the results here: with no branch
-        // around the loads the compiler knows exactly kUnroll loads are
-        // younger than the previous batch and waits for that batch alone.
-        auto fetch = [&](int at, int (&w)[kUnroll]) {
-            OffT bj[kUnroll];
+        // Element idx of adjP.  With int32 offsets adj holds fewer than 2^30
+        // ids (the engine's rule), so the byte offset fits 32 bits and the
+        // load takes a 32-bit offset from the scalar base: no sign extension
+        // and no 64-bit address arithmetic per lane.
+        auto load = [&](OffT idx) -> int {
+            if constexpr (sizeof(OffT) == 4)
+                return *(const int *)((const char *)adjP + ((uint32_t)idx << 2));
+            else
+                return adjP[idx];
+        };
+        auto bases = [&](int at, OffT (&bj)[kUnroll]) {
 #pragma unroll
             for (int k = 0; k < kUnroll; k++)
                 bj[k] = at + k * kWave < bend ? window_base(at + k * kWave) : (OffT)0;
+        };
+        // The loads of a batch lying entirely before bend.  Every lane loads
+        // and nothing reads the results here: with no branch around the
+        // loads the compiler knows exactly kUnroll loads are younger than the
+        // previous batch and waits for that batch alone.
+        auto fetch_full = [&](int at, int (&w)[kUnroll]) {
+            OffT bj[kUnroll];
+            bases(at, bj);
+#pragma unroll
+            for (int k = 0; k < kUnroll; k++)
+                w[k] = load(bj[k] + (OffT)(at + k * kWave + lane));
+        };
+        // The last batch: positions past bend read adjP[0] (which exists,
+        // since bend > 0) and are masked before the lookup.
+        auto fetch_tail = [&](int at, int (&w)[kUnroll]) {
+            OffT bj[kUnroll];
+            bases(at, bj);
 #pragma unroll
             for (int k = 0; k < kUnroll; k++) {
                 const int t = at + k * kWave + lane;
-                w[k] = adjP[t < bend ? bj[k] + (OffT)t : (OffT)0];
+                w[k] = load(t < bend ? bj[k] + (OffT)t : (OffT)0);
             }
         };
-        // Software pipeline: the next kUnroll windows' loads are in flight
-        // while the current ones are looked up.  Every batch inside the loop
-        // lies before bend; the last batch, looked up after the loop, may run
-        // past it and its positions there become kNoProbe.
+        // Software pipeline over batches of kUnroll windows: the next
+        // batch's loads are in flight while the current one is looked up.
+        constexpr int kBatch = kWave * kUnroll;
+        const int last = b + (bend - b - 1) / kBatch * kBatch;  // start of the last batch
         int wcur[kUnroll];
-        fetch(b, wcur);
-        int last = b;
-        for (int bn = b + kWave * kUnroll; bn < bend; bn += kWave * kUnroll) {
-            int wnext[kUnroll];
-            fetch(bn, wnext);
+        if (last == b) {
+            fetch_tail(b, wcur);
+        } else {
+            fetch_full(b, wcur);
+            for (int bn = b + kBatch; bn < last; bn += kBatch) {
+                int wnext[kUnroll];
+                fetch_full(bn, wnext);
+                lookup(wcur);
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++)
+                    wcur[k] = wnext[k];
+            }
+            int wtail[kUnroll];
+            fetch_tail(last, wtail);
             lookup(wcur);
 #pragma unroll
             for (int k = 0; k < kUnroll; k++)
-                wcur[k] = wnext[k];
-            last = bn;
+                wcur[k] = wtail[k];
         }
 #pragma unroll
         for (int k = 0; k < kUnroll; k++)
