@@ -375,20 +375,26 @@ __device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g
         return 0;
     // segment holding position b = number of lists ending at or before b
     int carry = (int)__popcll(__ballot(g.incl <= b));
+    // end and base of list `carry`, kept in scalar registers until it changes
+    int carry_incl = readlane(g.incl, carry);
+    OffT carry_base = readlane(g.base, carry);
     // Base of the window starting at position bk (< bend): adjP[base + t] is
-    // position t.  A window inside one list reads its base with a readlane;
-    // otherwise each position's list comes from the mark array and a max-scan.
+    // position t.  A window inside one list takes the cached base; otherwise
+    // each position's list comes from the mark array and a max-scan.
     auto window_base = [&](int bk) -> OffT {
-        if (readlane(g.incl, carry) >= bk + kWave)
-            return readlane(g.base, carry);
-        if (g.deg > 0 && g.excl >= bk && g.excl < bk + kWave)
-            mark[g.excl - bk] = lane;
+        if (carry_incl >= bk + kWave)
+            return carry_base;
+        const unsigned int off = (unsigned int)(g.excl - bk);
+        if (g.deg > 0 && off < (unsigned int)kWave)
+            mark[off] = lane;
         wave_sync();
         const int m = mark[lane];
         mark[lane] = 0;
         wave_sync();
         const int j = max(wave_scan_max(m), carry);
         carry = readlane(j, 63);
+        carry_incl = readlane(g.incl, carry);
+        carry_base = readlane(g.base, carry);
         return shfl_idx(g.base, j);
     };
     unsigned long long total = 0;  // wave-uniform hits
