@@ -664,15 +664,25 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     const int dW = (int)(a.endW[u] - sW);
     const int lg = table_lg(dH, a.wave_lg);
     const int size = 1 << lg;
+    // On an unpartitioned graph the hashed row and the walked row are one
+    // row: its first 64 ids, loaded for the table, also start the walk
+    // (one dependent load less per vertex).
+    const bool same_row = kTable && a.adjH == a.adjW && sH == sW && dH == dW;
+    int head = 0;
+    if constexpr (kTable)
+        head = lane < dH ? a.adjH[sH + (OffT)lane] : 0;
     bool linear = false;
     if constexpr (kTable) {
         for (int i = lane; i < size; i += kWave)
             tab[i] = kEmpty;
         wave_sync();
         bool failed = a.force_linear != 0;
-        if (!a.force_linear)
-            for (int i = lane; i < dH; i += kWave)
+        if (!a.force_linear) {
+            if (lane < dH)
+                failed |= cuckoo_insert(tab, lg - 1, head);
+            for (int i = lane + kWave; i < dH; i += kWave)
                 failed |= cuckoo_insert(tab, lg - 1, a.adjH[sH + (OffT)i]);
+        }
         wave_sync();
         linear = __ballot(failed) != 0;
         if (linear) {  // rebuild this vertex's table with linear probing
@@ -688,7 +698,8 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     unsigned long long wave_probes = 0;
     for (int c = 0; c < dW; c += kWave) {
         const bool valid = c + lane < dW;
-        const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)(c + lane)] : 0, valid, a.begP);
+        const int v = c == 0 && same_row ? head : (valid ? a.adjW[sW + (OffT)(c + lane)] : 0);
+        const Group<OffT> g = load_group(v, valid, a.begP);
         wave_probes += (unsigned long long)g.total;
         hits += probe_windows<OffT, false, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab, lg,
                                            linear, mark);
