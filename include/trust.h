@@ -288,6 +288,29 @@ void trust_csr_free(trust_csr *csr);
  */
 int trust_preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, trust_csr *out);
 
+/* Row orders of the reference's preprocessing tools.  All three share the
+ * ascending-degree sort and the orientation; only the final row order and ids
+ * differ, so every order counts the same triangles.
+ *   OUTDEGREE  preprocess_outdegree.cpp (and speedupIO/preprocess_out.cpp):
+ *              descending out-degree, first-touch relabel per class (:100-185)
+ *   PLAIN      otherPreprocess/preprocess.cpp:97-135,178: descending
+ *              out-degree, ids = positions, no relabel
+ *   INDEGREE   otherPreprocess/preprocess_indegree.cpp:131-156: the
+ *              ascending-degree order split stably into the classes
+ *              out-degree > 100, 2..100, < 2; ids = positions */
+#define TRUST_ORDER_OUTDEGREE 0
+#define TRUST_ORDER_PLAIN 1
+#define TRUST_ORDER_INDEGREE 2
+/* speedupIO/preprocess_out.cpp:101-126 deduplicates each adjacency list, so
+ * a self-loop adds 1 to its vertex's degree; the 1.mmio path adds 2
+ * (preprocess_outdegree.cpp:71-72).  The degree decides the sort, so the
+ * bytes differ when self-loops are present. */
+#define TRUST_PRE_SELF_LOOP_ONCE 1u
+/* trust_preprocess_edges with a row order and flags (above); any other flag
+ * bit is rejected. */
+int trust_preprocess_edges_ex(const int32_t *src, const int32_t *dst, int64_t m, int order,
+                              uint32_t flags, trust_csr *out);
+
 /* Tile (i,j) of an n x n hash partition (Preprocess/partition.cpp:72-121). */
 int trust_partition_csr(const trust_csr *csr, int n, trust_csr *tiles /* n*n */);
 /* Write partition{i}_{j}/{begin.bin,adjacent.bin,map,division} under folder

@@ -10,6 +10,13 @@ For each case:
                      oracle/_ref by oracle/Makefile
   part{n}/partition{i}_{j}/{begin.bin,adjacent.bin,map,division}
                      output of the reference's Preprocess/partition for n = 2, 3
+  speedup/{prop.txt,degree,BinEdgelist,begin.bin,adjacent.bin}
+                     output of Preprocess/speedupIO/TransformTxt2Binary (fed the
+                     pairs without the comment line, which that tool cannot
+                     skip) and speedupIO/preprocess_out
+  plain/{begin.bin,adjacent.bin,edge}, indegree/{begin.bin,adjacent.bin,edge}
+                     output of Preprocess/otherPreprocess/preprocess.cpp and
+                     preprocess_indegree.cpp on the same 1.mmio
   expect.json        V, E, and the triangle count from two sources independent
                      of this project: networkx (sum(triangles)/3) and the
                      brute-force bitset counter on the raw edge list
@@ -86,6 +93,38 @@ def run_ref(edges, workdir):
         os.remove(os.path.join(pd, "adjacent.bin"))
 
 
+def _run_allow_exit_abort(cmd, cwd, outputs):
+    """otherPreprocess/preprocess.cpp corrupts its heap in averagedegree()
+    (memset of sizeof(pointer) bytes, then unbounded reads of the
+    uninitialised counts, :136-152) and can abort at exit, after computeCSR
+    has written and closed its files.  Accept that abort when every output
+    exists."""
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True)
+    if r.returncode != 0 and not all(os.path.exists(os.path.join(cwd, f)) for f in outputs):
+        raise SystemExit(f"{cmd} failed with {r.returncode}")
+
+
+def run_variants(edges, workdir):
+    t2b, pout = oracle.ref_tool("TransformTxt2Binary"), oracle.ref_tool("preprocess_out")
+    plain, indeg = oracle.ref_tool("preprocess_plain"), oracle.ref_tool("preprocess_indegree")
+    if not (t2b and pout and plain and indeg):
+        raise SystemExit("oracle/_ref tools missing: run `make -C oracle` with the reference present")
+    sd = os.path.join(workdir, "speedup")
+    os.makedirs(sd)
+    with open(os.path.join(sd, "pairs.txt"), "w") as f:
+        for a, b in edges:
+            f.write(f"{a}\t{b}\n")
+    subprocess.run([t2b, "pairs.txt", str(len(edges))], cwd=sd, check=True, capture_output=True)
+    subprocess.run([pout], cwd=sd, check=True, capture_output=True)
+    os.remove(os.path.join(sd, "pairs.txt"))
+    for name, tool in (("plain", plain), ("indegree", indeg)):
+        d = os.path.join(workdir, name)
+        os.makedirs(d)
+        shutil.copy(os.path.join(workdir, "1.mmio"), d)
+        _run_allow_exit_abort([tool], d, ("begin.bin", "adjacent.bin", "edge"))
+        os.remove(os.path.join(d, "1.mmio"))
+
+
 def main():
     import networkx as nx
 
@@ -95,11 +134,12 @@ def main():
             shutil.rmtree(dst)
         work = tempfile.mkdtemp()
         run_ref(edges, work)
+        run_variants(edges, work)
         os.makedirs(dst)
         for fn in ("edges.txt", "begin.bin", "adjacent.bin"):
             shutil.copy(os.path.join(work, fn), dst)
-        for n in (2, 3):
-            shutil.copytree(os.path.join(work, f"part{n}"), os.path.join(dst, f"part{n}"))
+        for sub in ("part2", "part3", "speedup", "plain", "indegree"):
+            shutil.copytree(os.path.join(work, sub), os.path.join(dst, sub))
         shutil.rmtree(work)
         g = nx.Graph()
         g.add_edges_from((int(a), int(b)) for a, b in edges if a != b)

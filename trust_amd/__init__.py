@@ -8,7 +8,9 @@ counting path.
 Reference interfaces mirrored here (paths relative to the wzbxpy/TRUST root):
   triangle_count            Triangle_count, Without-graph-partition/TC.cuh:10
   triangle_count_partition  Triangle_count, With-graph-partition/TC.cuh:10
-  preprocess_edges          Preprocess/fromDirectToUndirect.cpp + preprocess_outdegree.cpp
+  preprocess_edges          Preprocess/fromDirectToUndirect.cpp + preprocess_outdegree.cpp;
+                            order= selects otherPreprocess/preprocess{,_indegree}.cpp,
+                            self_loop_once= the speedupIO/preprocess_out.cpp degree rule
   partition_csr / write     Preprocess/partition.cpp
 """
 from __future__ import annotations
@@ -28,6 +30,9 @@ ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2
 SCHED_DEGREE, SCHED_LEARNED, SCHED_AUTO = 0, 1, 2
 FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
 FLAG_HIP_GRAPH = 64
+ORDER_OUTDEGREE, ORDER_PLAIN, ORDER_INDEGREE = 0, 1, 2
+ORDERS = {"outdegree": ORDER_OUTDEGREE, "plain": ORDER_PLAIN, "indegree": ORDER_INDEGREE}
+PRE_SELF_LOOP_ONCE = 1
 # launch parameters the autotuner searches
 TUNED_FIELDS = ("threads", "wave_slots", "hub_threshold", "chunk")
 
@@ -131,6 +136,8 @@ SIGNATURES = {
     "trust_csr_validate": (C.c_int, [_P64, _P32, C.c_int64, C.c_int64]),
     "trust_csr_free": (None, [C.POINTER(Csr)]),
     "trust_preprocess_edges": (C.c_int, [_P32, _P32, C.c_int64, C.POINTER(Csr)]),
+    "trust_preprocess_edges_ex": (C.c_int, [_P32, _P32, C.c_int64, C.c_int, C.c_uint32,
+                                            C.POINTER(Csr)]),
     "trust_partition_csr": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Csr)]),
     "trust_partition_write": (C.c_int, [C.c_char_p, C.POINTER(Csr), C.c_int]),
     "trust_gen_rmat": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
@@ -237,12 +244,25 @@ def validate(beg, adj):
     _check(lib().trust_csr_validate(pb, pa, len(beg) - 1, len(adj)))
 
 
-def preprocess_edges(src, dst):
-    """Raw edge pairs -> (beg int64, adj int32) in the reference's oriented layout."""
+def preprocess_edges(src, dst, order="outdegree", self_loop_once=False):
+    """Raw edge pairs -> (beg int64, adj int32) in the reference's oriented layout.
+
+    order: "outdegree" (preprocess_outdegree.cpp, the default), "plain"
+    (otherPreprocess/preprocess.cpp) or "indegree"
+    (otherPreprocess/preprocess_indegree.cpp).  self_loop_once: a self-loop
+    adds 1 to the degree, as after speedupIO/preprocess_out.cpp's list dedup.
+    """
     src, ps = _i32(src)
     dst, pd = _i32(dst)
     c = Csr()
-    _check(lib().trust_preprocess_edges(ps, pd, len(src), C.byref(c)))
+    if order == "outdegree" and not self_loop_once:
+        _check(lib().trust_preprocess_edges(ps, pd, len(src), C.byref(c)))
+    else:
+        if order not in ORDERS:
+            raise ValueError(f"unknown order {order!r}")
+        flags = PRE_SELF_LOOP_ONCE if self_loop_once else 0
+        _check(lib().trust_preprocess_edges_ex(ps, pd, len(src), ORDERS[order], flags,
+                                               C.byref(c)))
     return _take_csr(c)
 
 

@@ -39,8 +39,16 @@ int64_t host_max_degree(const int64_t *beg, int64_t V);
 void radix_sort_u64(uint64_t *keys, uint64_t *tmp, size_t n, int key_bits);
 
 // Input contract (reference preprocessing), see preprocess.cpp.
+struct PreprocessOpts {
+    // >= 0: ids are already 0..n_fixed-1 and all are kept (a 1.mmio header
+    // or prop.txt fixes the vertex count); < 0: ids are compacted.
+    int64_t n_fixed = -1;
+    // what a self-loop adds to its vertex's degree before orientation
+    int self_loop_weight = 2;
+    int order = TRUST_ORDER_OUTDEGREE;
+};
 int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr &out,
-                     int64_t n_fixed = -1);
+                     const PreprocessOpts &o = PreprocessOpts());
 
 // A batch CSR: vbase[0] = 0, non-decreasing, vbase[ng] = V, and every edge
 // of graph g stays inside [vbase[g], vbase[g+1]) (batch.cpp, OpenMP).

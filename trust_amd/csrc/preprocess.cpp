@@ -29,6 +29,17 @@
 // first-touch relabel is computed as "rank of the first edge index at which
 // a vertex appears", which is exactly the order in which the reference's
 // sequential scan hands out ids.
+//
+// The reference's other preprocessing tools are the same pipeline with two
+// switches (PreprocessOpts):
+//
+//   * self_loop_weight: Preprocess/speedupIO/preprocess_out.cpp:101-126
+//     deduplicates every adjacency list, so a self-loop adds 1 to the degree
+//     where the 1.mmio path adds 2;
+//   * order: otherPreprocess/preprocess.cpp:178 keeps the descending
+//     out-degree order without the relabel (ids = positions), and
+//     otherPreprocess/preprocess_indegree.cpp:131-156 replaces that sort by
+//     a stable split of the ascending-degree order into the three classes.
 #include <omp.h>
 
 #include <algorithm>
@@ -96,10 +107,15 @@ struct SizeRec {
 };
 
 int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr &out,
-                     int64_t n_fixed)
+                     const PreprocessOpts &o)
 {
     if (m < 0 || (m > 0 && (!src || !dst)))
         return fail(TRUST_ERR_ARG, "bad edge list (m=%lld)", (long long)m);
+    if (o.order < TRUST_ORDER_OUTDEGREE || o.order > TRUST_ORDER_INDEGREE)
+        return fail(TRUST_ERR_ARG, "unknown row order %d", o.order);
+    if (o.self_loop_weight < 1 || o.self_loop_weight > 2)
+        return fail(TRUST_ERR_ARG, "self-loop weight must be 1 or 2");
+    const int64_t n_fixed = o.n_fixed;
     int64_t maxid = -1, neg = 0;
 #pragma omp parallel for reduction(max : maxid) reduction(+ : neg)
     for (int64_t e = 0; e < m; e++) {
@@ -153,11 +169,17 @@ int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr 
         if (e == 0 || keys[e] != keys[e - 1])
             keys[K++] = keys[e];
 
-    // ---- degree with self-loops counted twice (preprocess_outdegree.cpp:66-73)
+    // ---- degree; a self-loop counts twice on the 1.mmio path
+    // (preprocess_outdegree.cpp:66-73), once after speedupIO's list dedup
     std::vector<int64_t> deg((size_t)n, 0);
+    const int64_t loop_w = o.self_loop_weight;
 #pragma omp parallel for
     for (int64_t e = 0; e < K; e++) {
         int64_t a = (int64_t)(keys[e] >> 32), b = (int64_t)(keys[e] & 0xffffffffu);
+        if (a == b) {
+            __atomic_fetch_add(&deg[a], loop_w, __ATOMIC_RELAXED);
+            continue;
+        }
         __atomic_fetch_add(&deg[a], 1, __ATOMIC_RELAXED);
         __atomic_fetch_add(&deg[b], 1, __ATOMIC_RELAXED);
     }
@@ -184,18 +206,35 @@ int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr 
         __atomic_fetch_add(&outdeg[s], 1, __ATOMIC_RELAXED);
     }
 
-    // ---- descending out-degree order (:25-28,236) ----------------------
-    std::vector<SizeRec> s2((size_t)n);
-    for (int64_t i = 0; i < n; i++)
-        s2[i] = SizeRec{outdeg[s1[i].id], (int32_t)i};
-    std::sort(s2.begin(), s2.end(),
-              [](const SizeRec &a, const SizeRec &b) { return a.size > b.size; });
-    std::vector<int32_t> a2((size_t)n);  // vertex -> position p
+    std::vector<int32_t> a2((size_t)n);  // vertex -> row position p
+    if (o.order == TRUST_ORDER_INDEGREE) {
+        // ---- ascending-degree order split stably into the classes
+        // out-degree > 100, 2..100, < 2 (preprocess_indegree.cpp:131-156)
+        auto cls = [](int64_t d) { return d > 100 ? 0 : (d >= 2 ? 1 : 2); };
+        int64_t next[3] = {0, 0, 0};
+        for (int64_t i = 0; i < n; i++)
+            next[cls(outdeg[s1[i].id])]++;
+        next[2] = next[0] + next[1];
+        next[1] = next[0];
+        next[0] = 0;
+        for (int64_t i = 0; i < n; i++) {
+            const int32_t x = s1[i].id;
+            a2[x] = (int32_t)next[cls(outdeg[x])]++;
+        }
+        std::vector<SizeRec>().swap(s1);
+    } else {
+        // ---- descending out-degree order (:25-28,236) ------------------
+        std::vector<SizeRec> s2((size_t)n);
+        for (int64_t i = 0; i < n; i++)
+            s2[i] = SizeRec{outdeg[s1[i].id], (int32_t)i};
+        std::sort(s2.begin(), s2.end(),
+                  [](const SizeRec &a, const SizeRec &b) { return a.size > b.size; });
 #pragma omp parallel for
-    for (int64_t p = 0; p < n; p++)
-        a2[s1[s2[p].id].id] = (int32_t)p;
-    std::vector<SizeRec>().swap(s1);
-    std::vector<SizeRec>().swap(s2);
+        for (int64_t p = 0; p < n; p++)
+            a2[s1[s2[p].id].id] = (int32_t)p;
+        std::vector<SizeRec>().swap(s1);
+        std::vector<SizeRec>().swap(s2);
+    }
 
     // ---- out-lists by position, neighbours by descending original id ----
     // key = (position of source) << 32 | (n-1 - original id of target)
@@ -230,6 +269,13 @@ int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr 
     std::vector<int32_t>().swap(a2);
     for (int64_t p = 0; p < n; p++)  // rows with no entries
         begL[p + 1] = std::max(begL[p + 1], begL[p]);
+    if (o.order != TRUST_ORDER_OUTDEGREE) {  // ids are the row positions
+        out.V = n;
+        out.E = E;
+        out.beg.swap(begL);
+        out.adj.swap(L);
+        return TRUST_OK;
+    }
 
     // ---- class-preserving first-touch relabel (:100-185) ---------------
     int64_t k2 = -1, k3 = -1, cutoff = n;
@@ -308,7 +354,31 @@ extern "C" int trust_preprocess_edges(const int32_t *src, const int32_t *dst, in
     HostCsr g;
     int rc;
     try {
-        rc = preprocess_edges(src, dst, m, g, -1);
+        rc = preprocess_edges(src, dst, m, g);
+    } catch (const std::bad_alloc &) {
+        return fail(TRUST_ERR_OOM, "out of host memory preprocessing %lld pairs", (long long)m);
+    }
+    if (rc)
+        return rc;
+    return export_csr(std::move(g), out);
+}
+
+extern "C" int trust_preprocess_edges_ex(const int32_t *src, const int32_t *dst, int64_t m,
+                                         int order, uint32_t flags, trust_csr *out)
+{
+    using namespace trust;
+    if (!out)
+        return fail(TRUST_ERR_ARG, "null output");
+    memset(out, 0, sizeof(*out));
+    if (flags & ~TRUST_PRE_SELF_LOOP_ONCE)
+        return fail(TRUST_ERR_ARG, "unknown preprocessing flags 0x%x", flags);
+    PreprocessOpts o;
+    o.order = order;
+    o.self_loop_weight = (flags & TRUST_PRE_SELF_LOOP_ONCE) ? 1 : 2;
+    HostCsr g;
+    int rc;
+    try {
+        rc = preprocess_edges(src, dst, m, g, o);
     } catch (const std::bad_alloc &) {
         return fail(TRUST_ERR_OOM, "out of host memory preprocessing %lld pairs", (long long)m);
     }

@@ -5,9 +5,23 @@
 //       Preprocess/fromDirectToUndirect.cpp: text "u v" lines (lines not
 //       starting with a digit are skipped) -> compact 1-based ids, u<=v,
 //       deduplicated, written as "V V M" + "u v" lines.
-//   trust_tool preprocess [in=1.mmio] [outdir=.]
+//   trust_tool preprocess [in=1.mmio] [outdir=.] [--order outdegree|plain|indegree]
 //       Preprocess/preprocess_outdegree.cpp: 1.mmio -> begin.bin, adjacent.bin
-//       (byte-identical to the reference tool).
+//       (byte-identical to the reference tool).  --order plain is
+//       Preprocess/otherPreprocess/preprocess.cpp (also writes `edge`, the
+//       oriented (u,v) int32 pairs), --order indegree is
+//       otherPreprocess/preprocess_indegree.cpp (also writes `edge`, one int32
+//       per row: entries <= the row index found by the tool's binary search).
+//   trust_tool txt2bin <edgelist.txt> [edge_count_hint]
+//       Preprocess/speedupIO/TransformTxt2Binary.cpp: a whitespace-separated
+//       stream of id pairs -> prop.txt ("pairs vertices"), degree (int32 per
+//       compact id: endpoint occurrences), BinEdgelist (int32 pairs of compact
+//       ids, input order), written to the current directory.  A character
+//       that cannot start a number comments out the rest of its line (the
+//       reference tool stops making progress there).
+//   trust_tool preprocess-bin [dir=.]
+//       Preprocess/speedupIO/preprocess_out.cpp: dir/{prop.txt,BinEdgelist}
+//       -> dir/begin.bin, dir/adjacent.bin (byte-identical to the reference).
 //   trust_tool partition <n> [dir=.]
 //       Preprocess/partition.cpp: begin.bin/adjacent.bin in dir ->
 //       dir/partition{i}_{j}/{begin.bin,adjacent.bin,map,division}.
@@ -51,9 +65,10 @@ static bool read_pairs(const std::string &path, std::vector<int32_t> &a, std::ve
         std::istringstream ss(line);
         long long u, v;
         if (header && first) {
-            long long v2, m;
+            long long v2 = -1, m;
             ss >> u >> v2 >> m;
-            *hv = u;
+            hv[0] = u;
+            hv[1] = v2;
             first = false;
             continue;
         }
@@ -63,6 +78,182 @@ static bool read_pairs(const std::string &path, std::vector<int32_t> &a, std::ve
         b.push_back((int32_t)v);
     }
     return true;
+}
+
+// Whitespace-separated integer pairs, read in 64 MiB blocks (the fscanf
+// "%d%d" loop of TransformTxt2Binary.cpp:85-95, which pairs tokens across
+// line ends).  Ids must lie in [0, 2^31).
+static bool read_int_pairs(const std::string &path, std::vector<int32_t> &a,
+                           std::vector<int32_t> &b, std::string &err)
+{
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) {
+        err = "cannot open";
+        return false;
+    }
+    std::vector<char> buf((size_t)64 << 20);
+    size_t pos = 0, len = 0;
+    auto next = [&]() -> int {
+        if (pos == len) {
+            len = fread(buf.data(), 1, buf.size(), f);
+            pos = 0;
+            if (len == 0)
+                return EOF;
+        }
+        return (unsigned char)buf[pos++];
+    };
+    bool have_u = false;
+    int32_t u = 0;
+    int c = next();
+    while (c != EOF) {
+        if (c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f') {
+            c = next();
+            continue;
+        }
+        if (c == '-' || c == '+' || (c >= '0' && c <= '9')) {
+            const bool neg = c == '-';
+            if (c == '-' || c == '+')
+                c = next();
+            if (c < '0' || c > '9') {
+                err = "sign without digits";
+                fclose(f);
+                return false;
+            }
+            long long x = 0;
+            while (c >= '0' && c <= '9') {
+                x = x * 10 + (c - '0');
+                if (x > INT32_MAX) {
+                    err = "id above 2^31-1";
+                    fclose(f);
+                    return false;
+                }
+                c = next();
+            }
+            if (neg && x != 0) {
+                err = "negative id";
+                fclose(f);
+                return false;
+            }
+            if (!have_u) {
+                u = (int32_t)x;
+                have_u = true;
+            } else {
+                a.push_back(u);
+                b.push_back((int32_t)x);
+                have_u = false;
+            }
+            continue;
+        }
+        while (c != EOF && c != '\n')  // comment to the end of the line
+            c = next();
+    }
+    fclose(f);
+    if (have_u) {
+        err = "odd number of ids (the last pair has no second id)";
+        return false;
+    }
+    return true;
+}
+
+static int write_bytes(const std::string &path, const void *p, size_t n)
+{
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f || (n && fwrite(p, 1, n, f) != n)) {
+        if (f)
+            fclose(f);
+        fprintf(stderr, "trust_tool: cannot write %s\n", path.c_str());
+        return 1;
+    }
+    return fclose(f) == 0 ? 0 : 1;
+}
+
+static int cmd_txt2bin(int argc, char **argv)
+{
+    if (argc < 1)
+        return 2;
+    std::vector<int32_t> a, b;
+    if (argc > 1 && atoll(argv[1]) > 0) {
+        a.reserve((size_t)atoll(argv[1]));
+        b.reserve((size_t)atoll(argv[1]));
+    }
+    std::string err;
+    if (!read_int_pairs(argv[0], a, b, err)) {
+        fprintf(stderr, "trust_tool: %s: %s\n", argv[0], err.c_str());
+        return 1;
+    }
+    // ids that occur, compacted in ascending order (TransformTxt2Binary.cpp:97-121)
+    int64_t maxid = -1;
+    for (size_t e = 0; e < a.size(); e++)
+        maxid = std::max<int64_t>(maxid, std::max(a[e], b[e]));
+    std::vector<int64_t> occ((size_t)(maxid + 1), 0);
+    for (size_t e = 0; e < a.size(); e++) {
+        occ[a[e]]++;
+        occ[b[e]]++;
+    }
+    std::vector<int32_t> degree;
+    std::vector<int32_t> cid((size_t)(maxid + 1), -1);
+    for (int64_t x = 0; x <= maxid; x++)
+        if (occ[x]) {
+            if (occ[x] > INT32_MAX) {
+                fprintf(stderr, "trust_tool: id %lld occurs more than 2^31-1 times\n",
+                        (long long)x);
+                return 1;
+            }
+            cid[x] = (int32_t)degree.size();
+            degree.push_back((int32_t)occ[x]);
+        }
+    std::vector<int64_t>().swap(occ);
+    std::vector<int32_t> pairs(2 * a.size());
+    for (size_t e = 0; e < a.size(); e++) {
+        pairs[2 * e] = cid[a[e]];
+        pairs[2 * e + 1] = cid[b[e]];
+    }
+    char prop[64];
+    const int np = snprintf(prop, sizeof prop, "%zu %zu", a.size(), degree.size());
+    if (write_bytes("prop.txt", prop, (size_t)np) ||
+        write_bytes("degree", degree.data(), degree.size() * sizeof(int32_t)) ||
+        write_bytes("BinEdgelist", pairs.data(), pairs.size() * sizeof(int32_t)))
+        return 1;
+    printf("pairs=%zu vertices=%zu\n", a.size(), degree.size());
+    return 0;
+}
+
+static int cmd_preprocess_bin(int argc, char **argv)
+{
+    const std::string dir = argc > 0 ? argv[0] : ".";
+    long long m = -1, k = -1;
+    {
+        std::ifstream prop(dir + "/prop.txt");
+        if (!(prop >> m >> k) || m < 0 || k < 0 || k > INT32_MAX) {
+            fprintf(stderr, "trust_tool: %s/prop.txt: expected \"pairs vertices\"\n", dir.c_str());
+            return 1;
+        }
+    }
+    std::vector<int32_t> pairs((size_t)(2 * m));
+    FILE *f = fopen((dir + "/BinEdgelist").c_str(), "rb");
+    if (!f || fread(pairs.data(), sizeof(int32_t), pairs.size(), f) != pairs.size()) {
+        if (f)
+            fclose(f);
+        fprintf(stderr, "trust_tool: %s/BinEdgelist holds fewer than %lld pairs\n", dir.c_str(), m);
+        return 1;
+    }
+    fclose(f);
+    std::vector<int32_t> a((size_t)m), b((size_t)m);
+    for (long long e = 0; e < m; e++) {
+        a[e] = pairs[2 * e];
+        b[e] = pairs[2 * e + 1];
+    }
+    std::vector<int32_t>().swap(pairs);
+    PreprocessOpts o;
+    o.n_fixed = k;  // ids are already compact (preprocess_out.cpp:65-72)
+    o.self_loop_weight = 1;  // adjacency lists are deduplicated (:101-126)
+    HostCsr g;
+    int rc = preprocess_edges(a.data(), b.data(), (int64_t)m, g, o);
+    if (rc)
+        return die(rc);
+    trust_csr c{g.V, g.E, g.beg.data(), g.adj.data()};
+    rc = trust_csr_save(dir.c_str(), &c);
+    return rc ? die(rc) : 0;
 }
 
 static int cmd_undirect(int argc, char **argv)
@@ -105,13 +296,63 @@ static int cmd_undirect(int argc, char **argv)
     return 0;
 }
 
+// The `edge` side file of the otherPreprocess tools (preprocess.cpp:115-131,
+// preprocess_indegree.cpp:115-125), written from the final CSR.
+static int write_edge_file(const std::string &outdir, const HostCsr &g, int order)
+{
+    std::vector<int32_t> out;
+    if (order == TRUST_ORDER_PLAIN) {  // (row, id) for every entry
+        out.resize(2 * (size_t)g.E);
+        for (int64_t i = 0; i < g.V; i++)
+            for (int64_t e = g.beg[i]; e < g.beg[i + 1]; e++) {
+                out[2 * e] = (int32_t)i;
+                out[2 * e + 1] = g.adj[e];
+            }
+    } else {
+        // k = the tools' binary_search(32) over row lengths (:28-39), then per
+        // row the std::upper_bound of k in the row as stored (:122-123)
+        int64_t l = 0, r = g.V - 1;
+        while (l < r - 1) {
+            const int64_t mid = (l + r) >> 1;
+            if (g.beg[mid + 1] - g.beg[mid] >= 32)
+                l = mid;
+            else
+                r = mid;
+        }
+        const int32_t k = (int32_t)l;
+        out.resize((size_t)g.V);
+        for (int64_t i = 0; i < g.V; i++) {
+            const int32_t *row = g.adj.data() + g.beg[i], *end = g.adj.data() + g.beg[i + 1];
+            out[i] = (int32_t)(std::upper_bound(row, end, k) - row);
+        }
+    }
+    return write_bytes(outdir + "/edge", out.data(), out.size() * sizeof(int32_t));
+}
+
 static int cmd_preprocess(int argc, char **argv)
 {
-    std::string in = argc > 0 ? argv[0] : "1.mmio";
-    std::string outdir = argc > 1 ? argv[1] : ".";
+    std::vector<std::string> pos;
+    int order = TRUST_ORDER_OUTDEGREE;
+    for (int i = 0; i < argc; i++) {
+        if (!strcmp(argv[i], "--order") && i + 1 < argc) {
+            const std::string o = argv[++i];
+            if (o == "outdegree")
+                order = TRUST_ORDER_OUTDEGREE;
+            else if (o == "plain")
+                order = TRUST_ORDER_PLAIN;
+            else if (o == "indegree")
+                order = TRUST_ORDER_INDEGREE;
+            else
+                return 2;
+        } else {
+            pos.push_back(argv[i]);
+        }
+    }
+    std::string in = pos.size() > 0 ? pos[0] : "1.mmio";
+    std::string outdir = pos.size() > 1 ? pos[1] : ".";
     std::vector<int32_t> a, b;
-    long long hv = -1;
-    if (!read_pairs(in, a, b, true, &hv) || hv < 0) {
+    long long hv[2] = {-1, -1};
+    if (!read_pairs(in, a, b, true, hv) || hv[0] < 0 || hv[1] < 0) {
         fprintf(stderr, "trust_tool: cannot read %s\n", in.c_str());
         return 1;
     }
@@ -119,13 +360,20 @@ static int cmd_preprocess(int argc, char **argv)
         a[e]--;
         b[e]--;
     }
+    PreprocessOpts o;
+    // the vertex count is the header's first field in preprocess_outdegree.cpp:62
+    // and its second in otherPreprocess/preprocess*.cpp:60
+    o.n_fixed = order == TRUST_ORDER_OUTDEGREE ? hv[0] : hv[1];
+    o.order = order;
     HostCsr g;
-    int rc = preprocess_edges(a.data(), b.data(), (int64_t)a.size(), g, hv);
+    int rc = preprocess_edges(a.data(), b.data(), (int64_t)a.size(), g, o);
     if (rc)
         return die(rc);
     trust_csr c{g.V, g.E, g.beg.data(), g.adj.data()};
     rc = trust_csr_save(outdir.c_str(), &c);
-    return rc ? die(rc) : 0;
+    if (rc)
+        return die(rc);
+    return order == TRUST_ORDER_OUTDEGREE ? 0 : write_edge_file(outdir, g, order);
 }
 
 static int cmd_partition(int argc, char **argv)
@@ -208,6 +456,10 @@ int main(int argc, char **argv)
             rc = cmd_undirect(argc - 2, argv + 2);
         else if (cmd == "preprocess")
             rc = cmd_preprocess(argc - 2, argv + 2);
+        else if (cmd == "txt2bin")
+            rc = cmd_txt2bin(argc - 2, argv + 2);
+        else if (cmd == "preprocess-bin")
+            rc = cmd_preprocess_bin(argc - 2, argv + 2);
         else if (cmd == "partition")
             rc = cmd_partition(argc - 2, argv + 2);
         else if (cmd == "gen")
@@ -216,7 +468,9 @@ int main(int argc, char **argv)
     if (rc == 2)
         fprintf(stderr,
                 "usage: trust_tool undirect <edges.txt> [out]\n"
-                "       trust_tool preprocess [1.mmio] [outdir]\n"
+                "       trust_tool preprocess [1.mmio] [outdir] [--order outdegree|plain|indegree]\n"
+                "       trust_tool txt2bin <edges.txt> [edge_count_hint]\n"
+                "       trust_tool preprocess-bin [dir]\n"
                 "       trust_tool partition <n> [dir]\n"
                 "       trust_tool gen rmat <scale> <ef> <seed> <outdir>\n"
                 "       trust_tool gen chunglu <n> <m> <gamma> <maxdeg> <seed> <outdir>\n"
