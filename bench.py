@@ -28,6 +28,15 @@ Workloads (one JSON line each; the default is the headline):
             asynchronous tile copies; the total must equal the unpartitioned
             count.  One step = one streamed count (host-to-device copies
             included, tiles in pinned-free pageable host memory).
+  ingest    host only, no GPU: an R-MAT edge list (--ingest-scale, default
+            21) written as text, then the input tools timed -- the text path
+            (trust_tool undirect + preprocess, each --order) and the binary
+            fast path (txt2bin + preprocess-bin) -- beside the reference's
+            own tools built from source in oracle/_ref when present, with
+            every output compared byte for byte.
+
+--order plain|indegree (rmat) counts the graph in the row order of the
+reference's otherPreprocess tools instead of preprocess_outdegree's.
 
 Generation, preprocessing and the initial host-to-device copy happen before
 the timed region.  `value` is the whole job's throughput on one GPU.
@@ -49,6 +58,8 @@ METRICS = {
     "chung-lu": "triangle TEPS on 1 MI355X, autotuned degree split, exact count",
     "batch": "graphs/s on 1 MI355X, 10k-graph ER/BA batch, hipGraph replay, exact per-graph counts",
     "tiles": "triangle TEPS on 1 MI355X, 2D-partition tiles streamed from host, exact count",
+    "ingest": "edge-list ingest (text -> begin.bin/adjacent.bin) on the host, byte-identical "
+              "to the reference tools",
 }
 
 
@@ -181,8 +192,8 @@ def bench_rmat(args, T):
     t0 = time.perf_counter()
     src, dst = T.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
     t1 = time.perf_counter()
-    log(f"generated {len(src)} R-MAT pairs in {t1 - t0:.1f}s; preprocessing")
-    beg, adj = T.preprocess_edges(src, dst)
+    log(f"generated {len(src)} R-MAT pairs in {t1 - t0:.1f}s; preprocessing ({args.order} order)")
+    beg, adj = T.preprocess_edges(src, dst, order=args.order)
     del src, dst
     t2 = time.perf_counter()
     V, E = len(beg) - 1, len(adj)
@@ -228,11 +239,14 @@ def bench_rmat(args, T):
     r0, dev_s, cnt_s = head["r"], head["dev_s"], head["cnt_s"]
     W = r0.probes
     workload = f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}"
+    if args.order != "outdegree":
+        workload += f"_{args.order}"
     line = line_head(args, "rmat", E / dev_s / 1e9, "GTEPS", dev_s * 1e3)
     line["ms_per_step_wall"] = head["wall_ms"]
     line["config"] = {
         "workload": workload,
         "generator": "R-MAT a=0.57 b=0.19 c=0.19 d=0.05, reference preprocess contract",
+        "row_order": args.order,
         "V": V, "E": E, "W_probes": W, "triangles": r0.triangles,
         "schedule": "auto (learned cost model, judged against the degree split)"
         if args.schedule == "learned" else "degree",
@@ -450,6 +464,86 @@ def bench_tiles(args, T):
     return line
 
 
+def bench_ingest(args, T):
+    """Host-only timing of the input tools against the reference's tools."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import oracle
+
+    tool = os.path.join(T.BIN_DIR, "trust_tool")
+    src, dst = T.gen_rmat(args.ingest_scale, args.edge_factor, seed=args.seed)
+    m = len(src)
+    work = tempfile.mkdtemp(prefix="trust_ingest_")
+    try:
+        text = os.path.join(work, "e.txt")
+        np.savetxt(text, np.stack([src, dst], 1), fmt="%d", delimiter=" ")
+        del src, dst
+        log(f"ingest: {m} pairs, {os.path.getsize(text) / 1e6:.0f} MB of text in {work}")
+
+        def run(cmd, cwd, outputs, ok_abort=False):
+            t = time.perf_counter()
+            r = subprocess.run(cmd, cwd=cwd, capture_output=True)
+            dt = time.perf_counter() - t
+            if r.returncode != 0 and not (ok_abort and all(
+                    os.path.exists(os.path.join(cwd, f)) for f in outputs)):
+                raise SystemExit(f"{cmd} failed ({r.returncode}): {r.stderr[-400:]}")
+            return dt
+
+        steps = [  # (label, ours, reference tool, outputs, dir)
+            ("undirect", [tool, "undirect", text, "1.mmio"], ["fromDirectToUndirect", text],
+             ["1.mmio"], "txt"),
+            ("preprocess", [tool, "preprocess", "1.mmio", "."], ["preprocess"],
+             ["begin.bin", "adjacent.bin"], "txt"),
+            ("preprocess_plain", [tool, "preprocess", "../txt/1.mmio", ".", "--order", "plain"],
+             ["preprocess_plain"], ["begin.bin", "adjacent.bin", "edge"], "plain"),
+            ("preprocess_indegree", [tool, "preprocess", "../txt/1.mmio", ".", "--order", "indegree"],
+             ["preprocess_indegree"], ["begin.bin", "adjacent.bin", "edge"], "indegree"),
+            ("txt2bin", [tool, "txt2bin", text, str(m)], ["TransformTxt2Binary", text, str(m)],
+             ["prop.txt", "degree", "BinEdgelist"], "bin"),
+            ("preprocess_bin", [tool, "preprocess-bin", "."], ["preprocess_out"],
+             ["begin.bin", "adjacent.bin"], "bin"),
+        ]
+        have_ref = all(oracle.ref_tool(s[2][0]) for s in steps)
+        times, ref_times, identical = {}, {}, {}
+        for side in ("ours", "ref") if have_ref else ("ours",):
+            for label, ours, ref, outputs, sub in steps:
+                d = os.path.join(work, side, sub)
+                os.makedirs(d, exist_ok=True)
+                if sub in ("plain", "indegree") and side == "ref":
+                    shutil.copy(os.path.join(work, side, "txt", "1.mmio"), d)
+                cmd = ours if side == "ours" else [oracle.ref_tool(ref[0])] + ref[1:]
+                dt = run(cmd, d, outputs, ok_abort=side == "ref")
+                (times if side == "ours" else ref_times)[label] = dt
+                log(f"{side} {label}: {dt:.2f}s")
+        if have_ref:
+            for label, _, _, outputs, sub in steps:
+                identical[label] = all(
+                    open(os.path.join(work, "ours", sub, f), "rb").read() ==
+                    open(os.path.join(work, "ref", sub, f), "rb").read() for f in outputs)
+            if not all(identical.values()):
+                raise SystemExit(f"outputs differ from the reference tools: {identical}")
+        beg = np.fromfile(os.path.join(work, "ours", "txt", "begin.bin"), np.int64)
+        V, E = len(beg) - 1, int(beg[-1])
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    text_s = times["undirect"] + times["preprocess"]
+    line = line_head(args, "ingest", m / text_s / 1e6, "Mpairs/s", text_s * 1e3, dtype="int32")
+    line["n_gpus"] = 0
+    line["steps"], line["warmup"] = 1, 0
+    line["config"] = {
+        "workload": f"ingest_rmat{args.ingest_scale}_ef{args.edge_factor}_seed{args.seed}",
+        "pairs": m, "V": V, "E": E, "host_threads": os.cpu_count(),
+        "seconds": times,
+        "reference_seconds": ref_times or None,
+        "speedup_vs_reference": {k: ref_times[k] / times[k] for k in times} if ref_times else None,
+        "byte_identical": identical or None,
+        "parallelism": "host (OpenMP)",
+    }
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -471,6 +565,8 @@ def main():
     ap.add_argument("--parts", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--order", choices=["outdegree", "plain", "indegree"], default="outdegree")
+    ap.add_argument("--ingest-scale", type=int, default=21)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -483,7 +579,7 @@ def main():
 
     log(f"host: {os.cpu_count()} cpus, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}")
     fn = {"rmat": bench_rmat, "chung-lu": bench_chung_lu, "batch": bench_batch,
-          "tiles": bench_tiles}[args.workload]
+          "tiles": bench_tiles, "ingest": bench_ingest}[args.workload]
     print(json.dumps(fn(args, T)), flush=True)
 
 

@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 import trust_amd as T
-from conftest import CLOSED_FORMS, closed_form_csr, golden_cases, load_golden
+from conftest import CLOSED_FORMS, GOLDEN, closed_form_csr, golden_cases, load_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -167,6 +167,34 @@ def test_unsorted_general_csr(gpu):
         rng.shuffle(lst)
         na[nb[new]:nb[new + 1]] = lst
     assert T.count_csr(nb, na, device=gpu).triangles == want
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_golden_other_input_tools(gpu, name):
+    # the CSRs written by the reference's otherPreprocess and speedupIO tools
+    # (other row orders, self-loops weighed once): same triangles
+    _, _, meta = load_golden(name)
+    for sub in ("plain", "indegree", "speedup"):
+        d = os.path.join(GOLDEN, name, sub)
+        beg = np.fromfile(os.path.join(d, "begin.bin"), np.int64)
+        adj = np.fromfile(os.path.join(d, "adjacent.bin"), np.int32)
+        print(f"[gpu] golden {name}/{sub}", flush=True)
+        r = T.count_csr(beg, adj, device=gpu)
+        assert r.triangles == meta["triangles_networkx"]
+        assert r.probes == oracle.probes(beg, adj)
+
+
+@pytest.mark.parametrize("kw", [dict(order="plain"), dict(order="indegree"),
+                                dict(self_loop_once=True)])
+def test_reference_row_orders_rmat14(gpu, kw):
+    src, dst = T.gen_rmat(14, 16, seed=3)
+    beg, adj = T.preprocess_edges(src, dst, **kw)
+    want = oracle.count_hash(beg, adj)
+    assert want == oracle.count_merge(beg, adj)
+    print(f"[gpu] rmat14 {kw}", flush=True)
+    for ckw in ({}, dict(threads=1024, blocks=1024, chunk=1), dict(flags=T.FLAG_FORCE_SPILL)):
+        r = T.count_csr(beg, adj, device=gpu, **ckw)
+        assert r.triangles == want and r.probes == oracle.probes(beg, adj)
 
 
 def test_degenerate_graphs(gpu):

@@ -37,7 +37,6 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
-#include <sstream>
 #include <string>
 #include <vector>
 
@@ -51,49 +50,21 @@ static int die(int rc)
     return 1;
 }
 
-static bool read_pairs(const std::string &path, std::vector<int32_t> &a, std::vector<int32_t> &b,
-                       bool header, long long *hv)
-{
-    std::ifstream in(path);
-    if (!in)
-        return false;
-    std::string line;
-    bool first = true;
-    while (std::getline(in, line)) {
-        if (line.empty() || line[0] < '0' || line[0] > '9')
-            continue;
-        std::istringstream ss(line);
-        long long u, v;
-        if (header && first) {
-            long long v2 = -1, m;
-            ss >> u >> v2 >> m;
-            hv[0] = u;
-            hv[1] = v2;
-            first = false;
-            continue;
-        }
-        if (!(ss >> u >> v))
-            continue;
-        a.push_back((int32_t)u);
-        b.push_back((int32_t)v);
-    }
-    return true;
-}
-
-// Whitespace-separated integer pairs, read in 64 MiB blocks (the fscanf
-// "%d%d" loop of TransformTxt2Binary.cpp:85-95, which pairs tokens across
-// line ends).  Ids must lie in [0, 2^31).
-static bool read_int_pairs(const std::string &path, std::vector<int32_t> &a,
-                           std::vector<int32_t> &b, std::string &err)
-{
-    FILE *f = fopen(path.c_str(), "rb");
-    if (!f) {
-        err = "cannot open";
-        return false;
-    }
-    std::vector<char> buf((size_t)64 << 20);
+// A file read in 64 MiB blocks, one byte at a time.
+struct ByteReader {
+    FILE *f;
+    std::vector<char> buf;
     size_t pos = 0, len = 0;
-    auto next = [&]() -> int {
+    explicit ByteReader(const std::string &path)
+        : f(fopen(path.c_str(), "rb")), buf(f ? (size_t)64 << 20 : 0) {}
+    ~ByteReader()
+    {
+        if (f)
+            fclose(f);
+    }
+    bool ok() const { return f != nullptr; }
+    int next()
+    {
         if (pos == len) {
             len = fread(buf.data(), 1, buf.size(), f);
             pos = 0;
@@ -101,37 +72,113 @@ static bool read_int_pairs(const std::string &path, std::vector<int32_t> &a,
                 return EOF;
         }
         return (unsigned char)buf[pos++];
-    };
+    }
+};
+
+static bool is_blank(int c)
+{
+    return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
+}
+
+// Parses an optionally signed decimal starting at c; false if there are no
+// digits or the value leaves [-2^31, 2^31).  c ends on the first byte after.
+static bool parse_int(ByteReader &r, int &c, long long &x)
+{
+    const bool neg = c == '-';
+    if (c == '-' || c == '+')
+        c = r.next();
+    if (c < '0' || c > '9')
+        return false;
+    x = 0;
+    while (c >= '0' && c <= '9') {
+        x = x * 10 + (c - '0');
+        if (x > (long long)INT32_MAX + 1)
+            return false;
+        c = r.next();
+    }
+    if (neg)
+        x = -x;
+    return x <= INT32_MAX;
+}
+
+// Line-oriented edge text as the reference's readers take it
+// (fromDirectToUndirect.cpp:36-52, preprocess_outdegree.cpp:49-78): lines
+// that do not start with a digit are skipped, a line gives the first two
+// integers on it (a third, the 1.mmio weight column, is ignored), and with
+// `header` the first such line is "rows cols entries" (hv[0], hv[1]).
+static bool read_pairs(const std::string &path, std::vector<int32_t> &a, std::vector<int32_t> &b,
+                       bool header, long long *hv)
+{
+    ByteReader r(path);
+    if (!r.ok())
+        return false;
+    bool first = true;
+    int c = r.next();
+    while (c != EOF) {
+        if (c < '0' || c > '9') {  // not a data line
+            while (c != EOF && c != '\n')
+                c = r.next();
+            if (c == '\n')
+                c = r.next();
+            continue;
+        }
+        long long v[3] = {-1, -1, -1};
+        int nv = 0;
+        while (c != EOF && c != '\n') {
+            if (is_blank(c)) {
+                c = r.next();
+                continue;
+            }
+            long long x;
+            if (!((c >= '0' && c <= '9') || c == '-' || c == '+') || !parse_int(r, c, x)) {
+                while (c != EOF && c != '\n')  // the stream reader stops at this token
+                    c = r.next();
+                break;
+            }
+            if (nv < 3)
+                v[nv++] = x;
+        }
+        if (c == '\n')
+            c = r.next();
+        if (header && first) {
+            hv[0] = v[0];
+            hv[1] = v[1];
+            first = false;
+        } else if (nv >= 2) {
+            a.push_back((int32_t)v[0]);
+            b.push_back((int32_t)v[1]);
+        }
+    }
+    return true;
+}
+
+// Whitespace-separated integer pairs (the fscanf "%d%d" loop of
+// TransformTxt2Binary.cpp:85-95, which pairs tokens across line ends).  Ids
+// must lie in [0, 2^31).
+static bool read_int_pairs(const std::string &path, std::vector<int32_t> &a,
+                           std::vector<int32_t> &b, std::string &err)
+{
+    ByteReader r(path);
+    if (!r.ok()) {
+        err = "cannot open";
+        return false;
+    }
     bool have_u = false;
     int32_t u = 0;
-    int c = next();
+    int c = r.next();
     while (c != EOF) {
-        if (c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f') {
-            c = next();
+        if (is_blank(c) || c == '\n') {
+            c = r.next();
             continue;
         }
         if (c == '-' || c == '+' || (c >= '0' && c <= '9')) {
-            const bool neg = c == '-';
-            if (c == '-' || c == '+')
-                c = next();
-            if (c < '0' || c > '9') {
-                err = "sign without digits";
-                fclose(f);
+            long long x;
+            if (!parse_int(r, c, x)) {
+                err = "malformed or out-of-range id";
                 return false;
             }
-            long long x = 0;
-            while (c >= '0' && c <= '9') {
-                x = x * 10 + (c - '0');
-                if (x > INT32_MAX) {
-                    err = "id above 2^31-1";
-                    fclose(f);
-                    return false;
-                }
-                c = next();
-            }
-            if (neg && x != 0) {
+            if (x < 0) {
                 err = "negative id";
-                fclose(f);
                 return false;
             }
             if (!have_u) {
@@ -145,9 +192,8 @@ static bool read_int_pairs(const std::string &path, std::vector<int32_t> &a,
             continue;
         }
         while (c != EOF && c != '\n')  // comment to the end of the line
-            c = next();
+            c = r.next();
     }
-    fclose(f);
     if (have_u) {
         err = "odd number of ids (the last pair has no second id)";
         return false;
@@ -290,10 +336,29 @@ static int cmd_undirect(int argc, char **argv)
         return 1;
     }
     fprintf(f, "%lld %lld %zu\n", (long long)k, (long long)k, keys.size());
-    for (uint64_t x : keys)
-        fprintf(f, "%u %u\n", (unsigned)(x >> 32), (unsigned)(x & 0xffffffffu));
-    fclose(f);
-    return 0;
+    std::vector<char> obuf((size_t)1 << 24);
+    size_t n = 0;
+    auto put = [&](uint32_t x, char end) {
+        char tmp[12];
+        int t = 0;
+        do {
+            tmp[t++] = (char)('0' + x % 10);
+            x /= 10;
+        } while (x);
+        while (t)
+            obuf[n++] = tmp[--t];
+        obuf[n++] = end;
+    };
+    for (uint64_t x : keys) {
+        put((uint32_t)(x >> 32), ' ');
+        put((uint32_t)(x & 0xffffffffu), '\n');
+        if (n > obuf.size() - 32) {
+            fwrite(obuf.data(), 1, n, f);
+            n = 0;
+        }
+    }
+    fwrite(obuf.data(), 1, n, f);
+    return fclose(f) == 0 ? 0 : 1;
 }
 
 // The `edge` side file of the otherPreprocess tools (preprocess.cpp:115-131,
