@@ -31,9 +31,11 @@ Workloads (one JSON line each; the default is the headline):
   ingest    host only, no GPU: an R-MAT edge list (--ingest-scale, default
             21) written as text, then the input tools timed -- the text path
             (trust_tool undirect + preprocess, each --order) and the binary
-            fast path (txt2bin + preprocess-bin) -- beside the reference's
-            own tools built from source in oracle/_ref when present, with
-            every output compared byte for byte.
+            fast path (txt2bin + preprocess-bin) and the tile writer
+            (trust_tool partition, --parts n) -- beside the reference's own
+            tools built from source in oracle/_ref when present, with every
+            output compared byte for byte and each tool's peak anonymous
+            memory sampled from /proc.
 
 --order plain|indegree (rmat) counts the graph in the row order of the
 reference's otherPreprocess tools instead of preprocess_outdegree's.
@@ -482,13 +484,30 @@ def bench_ingest(args, T):
         del src, dst
         log(f"ingest: {m} pairs, {os.path.getsize(text) / 1e6:.0f} MB of text in {work}")
 
-        def run(cmd, cwd, outputs, ok_abort=False):
+        peaks = {}
+
+        def run(cmd, cwd, outputs, ok_abort=False, key=None):
+            # wall time, and the peak of the child's anonymous resident memory
             t = time.perf_counter()
-            r = subprocess.run(cmd, cwd=cwd, capture_output=True)
+            p = subprocess.Popen(cmd, cwd=cwd, stdout=subprocess.DEVNULL,
+                                 stderr=subprocess.PIPE)
+            peak = 0
+            while p.poll() is None:
+                try:
+                    with open(f"/proc/{p.pid}/status") as f:
+                        for ln in f:
+                            if ln.startswith("RssAnon:"):
+                                peak = max(peak, int(ln.split()[1]))
+                except OSError:
+                    pass
+                time.sleep(0.02)
             dt = time.perf_counter() - t
-            if r.returncode != 0 and not (ok_abort and all(
+            err = p.stderr.read().decode(errors="replace")
+            if p.returncode != 0 and not (ok_abort and all(
                     os.path.exists(os.path.join(cwd, f)) for f in outputs)):
-                raise SystemExit(f"{cmd} failed ({r.returncode}): {r.stderr[-400:]}")
+                raise SystemExit(f"{cmd} failed ({p.returncode}): {err[-400:]}")
+            if key:
+                peaks[key] = peak / 2**20  # GiB
             return dt
 
         steps = [  # (label, ours, reference tool, outputs, dir)
@@ -514,14 +533,33 @@ def bench_ingest(args, T):
                 if sub in ("plain", "indegree") and side == "ref":
                     shutil.copy(os.path.join(work, side, "txt", "1.mmio"), d)
                 cmd = ours if side == "ours" else [oracle.ref_tool(ref[0])] + ref[1:]
-                dt = run(cmd, d, outputs, ok_abort=side == "ref")
+                dt = run(cmd, d, outputs, ok_abort=side == "ref", key=f"{side} {label}")
                 (times if side == "ours" else ref_times)[label] = dt
-                log(f"{side} {label}: {dt:.2f}s")
+                log(f"{side} {label}: {dt:.2f}s, peak anon {peaks[f'{side} {label}']:.2f} GiB")
+        # tile writer (Preprocess/partition.cpp) on the preprocessed graph
+        n = args.parts
+        tile_files = [os.path.join(f"partition{i}_{j}", f) for i in range(n) for j in range(n)
+                      for f in ("begin.bin", "adjacent.bin", "map", "division")]
+        for side in ("ours", "ref") if have_ref and n <= 10 else ("ours",):
+            d = os.path.join(work, side, "tiles")
+            os.makedirs(d)
+            for f in ("begin.bin", "adjacent.bin"):
+                os.symlink(os.path.join(work, "ours", "txt", f), os.path.join(d, f))
+            if side == "ref":  # the reference tool needs the folders to exist
+                for i in range(n):
+                    for j in range(n):
+                        os.makedirs(os.path.join(d, f"partition{i}_{j}"))
+            cmd = [tool, "partition", str(n), "."] if side == "ours" else \
+                [oracle.ref_tool("partition"), str(n)]
+            dt = run(cmd, d, tile_files, key=f"{side} partition")
+            (times if side == "ours" else ref_times)["partition"] = dt
+            log(f"{side} partition n={n}: {dt:.2f}s, peak anon {peaks[f'{side} partition']:.2f} GiB")
         if have_ref:
-            for label, _, _, outputs, sub in steps:
-                identical[label] = all(
-                    open(os.path.join(work, "ours", sub, f), "rb").read() ==
-                    open(os.path.join(work, "ref", sub, f), "rb").read() for f in outputs)
+            for label, _, _, outputs, sub in steps + [("partition", 0, 0, tile_files, "tiles")]:
+                if label in ref_times:
+                    identical[label] = all(
+                        open(os.path.join(work, "ours", sub, f), "rb").read() ==
+                        open(os.path.join(work, "ref", sub, f), "rb").read() for f in outputs)
             if not all(identical.values()):
                 raise SystemExit(f"outputs differ from the reference tools: {identical}")
         beg = np.fromfile(os.path.join(work, "ours", "txt", "begin.bin"), np.int64)
@@ -535,8 +573,10 @@ def bench_ingest(args, T):
     line["config"] = {
         "workload": f"ingest_rmat{args.ingest_scale}_ef{args.edge_factor}_seed{args.seed}",
         "pairs": m, "V": V, "E": E, "host_threads": os.cpu_count(),
+        "partition_parts": args.parts,
         "seconds": times,
         "reference_seconds": ref_times or None,
+        "peak_anon_gib": {k: round(v, 3) for k, v in peaks.items()},
         "speedup_vs_reference": {k: ref_times[k] / times[k] for k in times} if ref_times else None,
         "byte_identical": identical or None,
         "parallelism": "host (OpenMP)",

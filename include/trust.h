@@ -316,6 +316,11 @@ int trust_partition_csr(const trust_csr *csr, int n, trust_csr *tiles /* n*n */)
 /* Write partition{i}_{j}/{begin.bin,adjacent.bin,map,division} under folder
  * (Preprocess/partition.cpp:101-165). */
 int trust_partition_write(const char *folder, const trust_csr *csr, int n);
+/* Same tile folders, written from <folder>/begin.bin + adjacent.bin mapped
+ * read-only from disk, one tile row (n tiles) in host memory at a time:
+ * the reference (partition.cpp:72-121) holds the whole graph and every tile
+ * in memory and stops at n <= 10 (:74-77); this takes n <= 1024. */
+int trust_partition_folder(const char *folder, int n);
 
 /* ---- seeded synthetic generators (edge lists, not yet preprocessed) --- */
 /* R-MAT: m = edge_factor << scale samples into src/dst (caller-allocated). */

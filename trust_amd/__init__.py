@@ -140,6 +140,7 @@ SIGNATURES = {
                                             C.POINTER(Csr)]),
     "trust_partition_csr": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Csr)]),
     "trust_partition_write": (C.c_int, [C.c_char_p, C.POINTER(Csr), C.c_int]),
+    "trust_partition_folder": (C.c_int, [C.c_char_p, C.c_int]),
     "trust_gen_rmat": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
                                  C.c_uint64, _P32, _P32]),
     "trust_gen_chung_lu": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_uint64,
@@ -272,6 +273,11 @@ def partition_csr(beg, adj, n: int):
     tiles = (Csr * (n * n))()
     _check(lib().trust_partition_csr(C.byref(c), n, tiles))
     return [_take_csr(tiles[k]) for k in range(n * n)]
+
+
+def partition_folder(folder: str, n: int):
+    """Tile folders written from folder/begin.bin + adjacent.bin, one tile row in memory."""
+    _check(lib().trust_partition_folder(folder.encode(), n))
 
 
 def partition_write(folder: str, beg, adj, n: int):

@@ -24,7 +24,9 @@
 //       -> dir/begin.bin, dir/adjacent.bin (byte-identical to the reference).
 //   trust_tool partition <n> [dir=.]
 //       Preprocess/partition.cpp: begin.bin/adjacent.bin in dir ->
-//       dir/partition{i}_{j}/{begin.bin,adjacent.bin,map,division}.
+//       dir/partition{i}_{j}/{begin.bin,adjacent.bin,map,division}; the
+//       input is mapped from disk and one tile row is held at a time, any
+//       n up to 1024 (the reference: whole graph in memory, n <= 10).
 //   trust_tool gen rmat <scale> <edge_factor> <seed> <outdir>
 //   trust_tool gen chunglu <n> <m> <gamma> <max_degree> <seed> <outdir>
 //   trust_tool gen er <n> <m> <seed> <outdir>
@@ -447,12 +449,7 @@ static int cmd_partition(int argc, char **argv)
         return 2;
     int n = atoi(argv[0]);
     std::string dir = argc > 1 ? argv[1] : ".";
-    trust_csr c;
-    int rc = trust_csr_load(dir.c_str(), &c);
-    if (rc)
-        return die(rc);
-    rc = trust_partition_write(dir.c_str(), &c, n);
-    trust_csr_free(&c);
+    int rc = trust_partition_folder(dir.c_str(), n);  // one tile row in memory at a time
     return rc ? die(rc) : 0;
 }
 
