@@ -38,6 +38,20 @@ extern "C" {
 #define TRUST_FLAG_WAVE_ONLY 8u    /* no workgroup tier: every vertex that fits a wave table uses one wave */
 #define TRUST_FLAG_LINEAR_PROBING 16u /* linear-probing tables instead of two-choice cuckoo buckets */
 #define TRUST_FLAG_HIP_GRAPH 64u   /* batch counts: replay plan + count as one captured hipGraph */
+/* Walks.  The reference walks every v in N+(u) from u's table
+ * (Without-graph-partition/TC.cu:330-359,510-544).  The short-side walk
+ * intersects each oriented edge u -> v (d+(u) >= 2) from the shorter side:
+ * N+(v) through u's table when d+(v) <= d+(u), else N+(u) through v's table.
+ * Same triangles, min(d+(u), d+(v)) probes per edge.  It needs the in-walk
+ * rows (for every v, the u -> v with 2 <= d+(u) < d+(v)), a per-graph index
+ * that a context builds once, the first time a count uses it; that call
+ * reports the build in trust_result.index_seconds, outside seconds.
+ * Without either flag an unpartitioned count takes the short-side walk when
+ * its probes are at most 45% of the reference walk's (a pass over the graph,
+ * also once per graph and reported the same way); partitioned counts always
+ * take the reference walk.  Both flags together are rejected. */
+#define TRUST_FLAG_REFERENCE_WALK 128u
+#define TRUST_FLAG_SHORT_WALK 256u
 /* Any other bit in trust_params.flags is rejected with TRUST_ERR_ARG. */
 
 /*
@@ -82,7 +96,9 @@ typedef struct trust_result {
     int32_t hub_slots;     /* LDS hub-table slots per workgroup */
     int32_t wave_slots;    /* LDS table slots per wavefront */
     int32_t schedule_used; /* TRUST_SCHED_* that ran (LEARNED falls back to DEGREE without a model) */
-    int32_t reserved0;
+    int32_t walk_used;     /* 0: the reference walk, 1: the short-side walk */
+    double index_seconds;  /* one-time per-graph work this call did before its timed region:
+                              walk statistics and the in-walk rows (0 once built) */
 } trust_result;
 
 /* Library-allocated CSR (trust_csr_free releases it). */

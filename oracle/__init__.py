@@ -12,6 +12,8 @@ wzbxpy/TRUST root):
   count_brute       no reference counterpart (O(n^3/64) bitset)
   count_tile_job    With-graph-partition/TC.cu:630-973, 985-994
   probes            number of linear_search calls (TC.cu:354,534)
+  probes_short      probes of the product's short-side walk (no reference
+                    counterpart; min(d+(u), d+(v)) per oriented edge)
   features          per-vertex d+(u), sum d+(v), max d+(v)
   mlp_*             numpy float64 forward/backward of the cost model
                     (no reference counterpart)
@@ -50,6 +52,8 @@ def lib():
                                             C.c_int64, C.c_int]
         h.oracle_probes.restype = C.c_uint64
         h.oracle_probes.argtypes = [_P64, _P32, C.c_int64, _P64]
+        h.oracle_probes_short.restype = C.c_uint64
+        h.oracle_probes_short.argtypes = [_P64, _P32, C.c_int64, _P64]
         h.oracle_features.restype = None
         h.oracle_features.argtypes = [_P64, _P32, C.c_int64, C.POINTER(C.c_double)]
         h.oracle_count_hash_sample.restype = C.c_uint64
@@ -129,6 +133,15 @@ def probes(beg, adj, per_vertex: bool = False):
     V = len(beg) - 1
     out = np.zeros(V, np.int64) if per_vertex else None
     w = lib().oracle_probes(pb, pa, V, out.ctypes.data_as(_P64) if per_vertex else None)
+    return (int(w), out) if per_vertex else int(w)
+
+
+def probes_short(beg, adj, per_vertex: bool = False):
+    beg, pb = _i64(beg)
+    adj, pa = _i32(adj)
+    V = len(beg) - 1
+    out = np.zeros(V, np.int64) if per_vertex else None
+    w = lib().oracle_probes_short(pb, pa, V, out.ctypes.data_as(_P64) if per_vertex else None)
     return (int(w), out) if per_vertex else int(w)
 
 

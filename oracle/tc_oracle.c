@@ -36,6 +36,10 @@
  *   oracle_probes       W = sum over processed u of sum_{v in N+(u)} d+(v):
  *       the number of linear_search calls the reference makes
  *       (Without-graph-partition/TC.cu:354,534).
+ *   oracle_probes_short  the probes of the product's short-side walk (no
+ *       reference counterpart; the count is the reference's): each oriented
+ *       edge u -> v with d+(u) >= 2 costs min(d+(u), d+(v)) probes, charged
+ *       to u when d+(v) <= d+(u) and to v otherwise.
  *   oracle_features     the learned scheduler's per-vertex features
  *       d+(u), sum_{v in N+(u)} d+(v), max_{v in N+(u)} d+(v).
  *   oracle_count_hash_batch  oracle_count_hash per graph of a batch (graph
@@ -315,6 +319,31 @@ uint64_t oracle_probes(const int64_t *beg, const int32_t *adj, int64_t V, int64_
         if (per_vertex)
             per_vertex[u] = w;
         total += (uint64_t)w;
+    }
+    return total;
+}
+
+uint64_t oracle_probes_short(const int64_t *beg, const int32_t *adj, int64_t V,
+                             int64_t *per_vertex)
+{
+    if (per_vertex)
+        memset(per_vertex, 0, sizeof(int64_t) * (size_t)V);
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : total)
+    for (int64_t u = 0; u < V; u++) {
+        const int64_t du = beg[u + 1] - beg[u];
+        if (du < ORACLE_MIN_DEGREE)
+            continue;
+        for (int64_t e = beg[u]; e < beg[u + 1]; e++) {
+            const int64_t v = adj[e], dv = beg[v + 1] - beg[v];
+            const int64_t w = dv <= du ? dv : du;
+            total += (uint64_t)w;
+            if (per_vertex) {
+                int64_t *slot = per_vertex + (dv <= du ? u : v);
+#pragma omp atomic
+                *slot += w;
+            }
+        }
     }
     return total;
 }

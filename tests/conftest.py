@@ -91,6 +91,25 @@ CLOSED_FORMS = {
 }
 
 
+# Share of the reference walk's probes at or below which a count takes the
+# short-side walk by default (trust_amd/csrc/tc_kernels.h kShortWalkShare).
+SHORT_WALK_SHARE = 0.45
+
+
+def expected_probes(beg, adj, flags=0, per_vertex=False):
+    """Probes the GPU count makes: the short-side walk's when forced, or when
+    it makes at most SHORT_WALK_SHARE of the reference walk's; else the
+    reference walk's."""
+    import oracle
+    import trust_amd as T
+    ref, short = oracle.probes(beg, adj), oracle.probes_short(beg, adj)
+    use_short = bool(flags & T.FLAG_SHORT_WALK) or (
+        not flags & T.FLAG_REFERENCE_WALK and short <= SHORT_WALK_SHARE * ref)
+    if per_vertex:
+        return (oracle.probes_short if use_short else oracle.probes)(beg, adj, per_vertex=True)
+    return short if use_short else ref
+
+
 def closed_form_csr(name):
     import trust_amd as T
 

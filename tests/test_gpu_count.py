@@ -13,7 +13,8 @@ import pytest
 
 import oracle
 import trust_amd as T
-from conftest import CLOSED_FORMS, GOLDEN, closed_form_csr, golden_cases, load_golden
+from conftest import (CLOSED_FORMS, GOLDEN, closed_form_csr, expected_probes, golden_cases,
+                      load_golden)
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +25,8 @@ def rmat16():
     beg, adj = T.rmat_csr(16, 16, seed=1)
     want = oracle.count_hash(beg, adj)
     assert want == oracle.count_merge(beg, adj)
-    return beg, adj, want, oracle.probes(beg, adj)
+    # probes of the walk a default count takes (the rule in expected_probes)
+    return beg, adj, want, expected_probes(beg, adj)
 
 
 @pytest.fixture(scope="module")
@@ -48,9 +50,13 @@ def test_closed_forms(gpu, name):
 def test_golden(gpu, name):
     beg, adj, meta = load_golden(name)
     print(f"[gpu] golden {name}", flush=True)
-    r = T.count_csr(beg, adj, device=gpu)
-    assert r.triangles == meta["triangles_networkx"]
-    assert r.probes == oracle.probes(beg, adj)
+    for fl in (0, T.FLAG_REFERENCE_WALK, T.FLAG_SHORT_WALK):
+        r = T.count_csr(beg, adj, device=gpu, flags=fl)
+        assert r.triangles == meta["triangles_networkx"]
+        assert r.probes == expected_probes(beg, adj, fl)
+        short = oracle.probes_short(beg, adj) <= 0.45 * oracle.probes(beg, adj)
+        want_walk = 1 if fl == T.FLAG_SHORT_WALK else 0 if fl == T.FLAG_REFERENCE_WALK else int(short)
+        assert r.walk_used == want_walk
 
 
 def test_rmat16_default(ctx16, rmat16):
@@ -78,16 +84,66 @@ def test_rmat16_threads(ctx16, rmat16, threads):
     dict(flags=T.FLAG_FORCE_SPILL, threads=64, blocks=3),
     dict(flags=T.FLAG_LINEAR_PROBING), dict(flags=T.FLAG_LINEAR_PROBING | T.FLAG_FORCE_SPILL),
     dict(flags=T.FLAG_LINEAR_PROBING, hub_threshold=0),
+    dict(flags=T.FLAG_REFERENCE_WALK), dict(flags=T.FLAG_REFERENCE_WALK | T.FLAG_FORCE_SPILL),
+    dict(flags=T.FLAG_SHORT_WALK), dict(flags=T.FLAG_SHORT_WALK | T.FLAG_FORCE_SPILL),
+    dict(flags=T.FLAG_SHORT_WALK, threads=1024, blocks=1024, chunk=1),
+    dict(flags=T.FLAG_SHORT_WALK, threads=64, blocks=7, chunk=3),
+    dict(flags=T.FLAG_SHORT_WALK, hub_threshold=0), dict(flags=T.FLAG_SHORT_WALK, hub_threshold=1 << 30),
+    dict(flags=T.FLAG_SHORT_WALK | T.FLAG_WAVE_ONLY, wave_slots=2048, threads=256),
+    dict(flags=T.FLAG_SHORT_WALK | T.FLAG_LINEAR_PROBING),
+    dict(flags=T.FLAG_SHORT_WALK, lds_budget=4096, threads=64, wave_slots=256),
 ])
 def test_rmat16_invariance(ctx16, rmat16, kw):
     print(f"[gpu] rmat16 {kw}", flush=True)
     r = ctx16.count(**kw)
     assert r.triangles == rmat16[2]
-    assert r.probes == rmat16[3]
+    assert r.probes == expected_probes(rmat16[0], rmat16[1], kw.get("flags", 0))
+
+
+def test_short_side_walk_saves_probes(ctx16, rmat16):
+    beg, adj, want, _ = rmat16
+    ref = ctx16.count(flags=T.FLAG_REFERENCE_WALK)
+    short = ctx16.count(flags=T.FLAG_SHORT_WALK)
+    again = ctx16.count(flags=T.FLAG_SHORT_WALK)
+    print(f"[gpu] rmat16 probes reference {ref.probes} short-side {short.probes}; "
+          f"count {ref.count_seconds * 1e3:.3f} ms -> {short.count_seconds * 1e3:.3f} ms, "
+          f"index {short.index_seconds * 1e3:.3f} ms", flush=True)
+    assert ref.triangles == short.triangles == again.triangles == want
+    assert short.probes == oracle.probes_short(beg, adj) < ref.probes == oracle.probes(beg, adj)
+    assert short.walk_used == 1 and ref.walk_used == 0
+    assert again.index_seconds == 0.0  # the in-walk rows are built once per graph
+
+
+def test_walk_flags_together_rejected(ctx16):
+    with pytest.raises(T.TrustError):
+        ctx16.count(flags=T.FLAG_REFERENCE_WALK | T.FLAG_SHORT_WALK)
+    with pytest.raises(T.TrustError):
+        T.count_tiles(T.partition_csr(*T.rmat_csr(8, 8, seed=1), 2), 2,
+                      flags=T.FLAG_SHORT_WALK)
+
+
+def test_default_walk_choice_rmat21(gpu):
+    # R-MAT 21: the short side makes ~42% of the reference walk's probes, so a
+    # default count takes it; the reference walk must find the same triangles
+    beg, adj = T.rmat_csr(21, 16, seed=1)
+    ref_w, short_w = oracle.probes(beg, adj), oracle.probes_short(beg, adj)
+    assert short_w <= 0.45 * ref_w
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        auto = c.count()
+        ref = c.count(flags=T.FLAG_REFERENCE_WALK)
+        print(f"[gpu] rmat21 auto walk {auto.walk_used}: {auto.count_seconds * 1e3:.2f} ms vs "
+              f"reference {ref.count_seconds * 1e3:.2f} ms, index {auto.index_seconds * 1e3:.1f} ms",
+              flush=True)
+        assert auto.walk_used == 1 and auto.probes == short_w
+        assert ref.walk_used == 0 and ref.probes == ref_w
+        assert auto.triangles == ref.triangles
+        gauto, gref = c.gather(), c.gather(flags=T.FLAG_REFERENCE_WALK)
+        assert gauto.probes == short_w and gref.probes == ref_w
 
 
 def test_unknown_flags_rejected(ctx16):
-    for bad in (32, 128, 1 << 31):
+    for bad in (32, 512, 1 << 31):
         with pytest.raises(T.TrustError):
             ctx16.count(flags=bad)
 
@@ -114,10 +170,11 @@ def test_repeat_is_deterministic(ctx16, rmat16):
 
 def test_instrumented_probes_match_oracle(ctx16, rmat16):
     beg, adj, want, w = rmat16
-    cyc, prb, r = ctx16.profile_vertices()
-    assert r.triangles == want
-    _, per = oracle.probes(beg, adj, per_vertex=True)
-    assert np.array_equal(prb.astype(np.int64), per)
+    for fl in (0, T.FLAG_SHORT_WALK, T.FLAG_REFERENCE_WALK):
+        cyc, prb, r = ctx16.profile_vertices(flags=fl)
+        assert r.triangles == want
+        _, per = expected_probes(beg, adj, fl, per_vertex=True)
+        assert np.array_equal(prb.astype(np.int64), per)
     d = np.diff(beg)
     assert np.all(cyc[d >= 2] > 0) and np.all(cyc[d < 2] == 0)
 
@@ -179,9 +236,10 @@ def test_golden_other_input_tools(gpu, name):
         beg = np.fromfile(os.path.join(d, "begin.bin"), np.int64)
         adj = np.fromfile(os.path.join(d, "adjacent.bin"), np.int32)
         print(f"[gpu] golden {name}/{sub}", flush=True)
-        r = T.count_csr(beg, adj, device=gpu)
-        assert r.triangles == meta["triangles_networkx"]
-        assert r.probes == oracle.probes(beg, adj)
+        for fl in (0, T.FLAG_SHORT_WALK):
+            r = T.count_csr(beg, adj, device=gpu, flags=fl)
+            assert r.triangles == meta["triangles_networkx"]
+            assert r.probes == expected_probes(beg, adj, fl)
 
 
 @pytest.mark.parametrize("kw", [dict(order="plain"), dict(order="indegree"),
@@ -192,9 +250,10 @@ def test_reference_row_orders_rmat14(gpu, kw):
     want = oracle.count_hash(beg, adj)
     assert want == oracle.count_merge(beg, adj)
     print(f"[gpu] rmat14 {kw}", flush=True)
-    for ckw in ({}, dict(threads=1024, blocks=1024, chunk=1), dict(flags=T.FLAG_FORCE_SPILL)):
+    for ckw in ({}, dict(threads=1024, blocks=1024, chunk=1), dict(flags=T.FLAG_FORCE_SPILL),
+                dict(flags=T.FLAG_SHORT_WALK)):
         r = T.count_csr(beg, adj, device=gpu, **ckw)
-        assert r.triangles == want and r.probes == oracle.probes(beg, adj)
+        assert r.triangles == want and r.probes == expected_probes(beg, adj, ckw.get("flags", 0))
 
 
 def test_degenerate_graphs(gpu):

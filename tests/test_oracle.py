@@ -82,6 +82,56 @@ def test_probes_per_vertex_sum_and_skip():
     assert np.all(per[d < 2] == 0)  # vertices with d+ < 2 are skipped
 
 
+def test_probes_short_closed_form_complete_graph():
+    # in K_n's transitive tournament every edge u -> v has d+(v) < d+(u), so
+    # the short side is always the reference's side: W is C(n,3) again
+    for name, n in (("K10", 10), ("K50", 50)):
+        beg, adj, want = closed_form_csr(name)
+        assert oracle.probes_short(beg, adj) == want
+
+
+def _explicit_short_walk(beg, adj, in_ties=False):
+    """Brute force: build each vertex's two-part walk explicitly (out-part
+    with d+(v) <= d+(u), in-part = lower-d+ in-neighbours with d+ >= 2), probe
+    every listed N+(x) against a set, return (triangles, probes per vertex).
+    in_ties=True plants a mistake: the in-part also takes equal-d+
+    in-neighbours, so an edge between equal out-degrees is walked twice."""
+    d = np.diff(beg)
+    V = len(d)
+    ins = [[] for _ in range(V)]
+    for u in range(V):
+        for v in adj[beg[u]:beg[u + 1]]:
+            ins[v].append(u)
+    tri, per = 0, np.zeros(V, np.int64)
+    for o in range(V):
+        if d[o] < 2:
+            continue
+        tab = set(adj[beg[o]:beg[o + 1]].tolist())
+        walk = [v for v in adj[beg[o]:beg[o + 1]] if d[v] <= d[o]]
+        walk += [w for w in ins[o] if 2 <= d[w] and (d[w] < d[o] or (in_ties and d[w] == d[o]))]
+        for x in walk:
+            lst = adj[beg[x]:beg[x + 1]]
+            per[o] += len(lst)
+            tri += sum(1 for y in lst.tolist() if y in tab)
+    return tri, per
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_probes_short_against_explicit_walk(seed):
+    beg, adj = T.rmat_csr(9, 8, seed=seed + 20)
+    tri, per = _explicit_short_walk(beg, adj)
+    assert tri == oracle.count_hash(beg, adj)  # the short-side walk finds every triangle once
+    w, per_o = oracle.probes_short(beg, adj, per_vertex=True)
+    assert np.array_equal(per_o, per) and w == per.sum()
+    assert w < oracle.probes(beg, adj)  # R-MAT: strictly fewer probes than the reference walk
+
+
+def test_short_walk_tie_rule_mistake_is_caught():
+    beg, adj = T.rmat_csr(9, 8, seed=21)
+    bad, _ = _explicit_short_walk(beg, adj, in_ties=True)
+    assert bad > oracle.count_hash(beg, adj)
+
+
 def test_features_closed_form():
     # transitive tournament: the vertex with d+ = k points at vertices with
     # d+ = k-1..0, so sum = k(k-1)/2 and max = k-1.

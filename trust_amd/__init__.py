@@ -30,6 +30,8 @@ ERR_ARG, ERR_IO, ERR_FORMAT, ERR_OOM, ERR_HIP, ERR_OVERFLOW, ERR_NODEV = (-1, -2
 SCHED_DEGREE, SCHED_LEARNED, SCHED_AUTO = 0, 1, 2
 FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_PROBING = 1, 2, 4, 8, 16
 FLAG_HIP_GRAPH = 64
+FLAG_REFERENCE_WALK = 128
+FLAG_SHORT_WALK = 256
 ORDER_OUTDEGREE, ORDER_PLAIN, ORDER_INDEGREE = 0, 1, 2
 ORDERS = {"outdegree": ORDER_OUTDEGREE, "plain": ORDER_PLAIN, "indegree": ORDER_INDEGREE}
 PRE_SELF_LOOP_ONCE = 1
@@ -60,7 +62,8 @@ class Result(C.Structure):
         ("alg_bytes", C.c_double), ("n_spill", C.c_int64), ("n_hub", C.c_int64),
         ("n_wave", C.c_int64), ("threads", C.c_int32), ("blocks", C.c_int32),
         ("hub_slots", C.c_int32), ("wave_slots", C.c_int32),
-        ("schedule_used", C.c_int32), ("reserved0", C.c_int32),
+        ("schedule_used", C.c_int32), ("walk_used", C.c_int32),
+        ("index_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:

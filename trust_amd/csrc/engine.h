@@ -48,6 +48,9 @@ struct CsrViews {
     const int32_t *adjW;
     const void *begP;
     const int32_t *adjP;
+    // in-walk rows of the short-side walk (null: the reference's walk)
+    const void *begI = nullptr;
+    const int32_t *adjI = nullptr;
 };
 
 // Shape of one counting launch.
@@ -84,6 +87,12 @@ struct Ctx {
     double h2d_seconds = 0;
     double learned_plan_seconds = 0;
     DevBuf staging, list, block_counts, ctl, spill, instr_cycles, instr_probes;
+    // short-side walk: probe totals of both walks and the in-walk rows, a
+    // per-graph index made the first time a count needs it
+    DevBuf side_cnt, side_bsum, side_beg, side_adj, side_stats;
+    bool side_stats_ready = false, side_built = false;
+    uint64_t side_w_ref = 0, side_w_short = 0;
+    int64_t side_in_entries = 0;  // entries of the in-walk rows
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     bool mlp_ready = false;
@@ -112,6 +121,8 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
                    LaunchPlan &L, int mode = kModeCount);
 int check_params(const trust_params &p);
 double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64);
+// The same for the short-side walk, whose in-walk rows hold I entries.
+double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t I);
 
 template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
@@ -128,6 +139,8 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
 struct ResidentRun {
     LaunchPlan L;
     bool learned = false;
+    bool side = false;           // short-side walk
+    double index_seconds = 0;    // one-time index work done while preparing
 };
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
 // Releases the captured batch graph (after a new load or at teardown).

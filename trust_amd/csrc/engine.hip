@@ -222,6 +222,8 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.ctl = ctl_dev;
     a.instr_cycles = instr_cycles;
     a.instr_probes = instr_probes;
+    a.begI = (const OffT *)views.begI;
+    a.adjI = views.adjI;
     if (ctx->batch_ngraphs > 0) {  // read by the batch mode only
         const int64_t ng = ctx->batch_ngraphs;
         a.graph_vbase = (const long long *)ctx->batch_vbase.p;
@@ -269,6 +271,7 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
         pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
         pa.tier_override = nullptr;
+        pa.begI = (const OffT *)views.begI;
         pa.list = (int32_t *)ctx->list.p;
         pa.block_counts = (long long *)ctx->block_counts.p;
         pa.per_block = per_block;
@@ -318,9 +321,12 @@ int check_params(const trust_params &p)
         p.schedule != TRUST_SCHED_AUTO)
         return fail(TRUST_ERR_ARG, "unknown schedule %d", p.schedule);
     const uint32_t known = TRUST_FLAG_FORCE_SPILL | TRUST_FLAG_OFFSETS64 | TRUST_FLAG_NO_VALIDATE |
-                           TRUST_FLAG_WAVE_ONLY | TRUST_FLAG_LINEAR_PROBING | TRUST_FLAG_HIP_GRAPH;
+                           TRUST_FLAG_WAVE_ONLY | TRUST_FLAG_LINEAR_PROBING | TRUST_FLAG_HIP_GRAPH |
+                           TRUST_FLAG_REFERENCE_WALK | TRUST_FLAG_SHORT_WALK;
     if (p.flags & ~known)
         return fail(TRUST_ERR_ARG, "unknown flag bits 0x%x", p.flags & ~known);
+    if ((p.flags & TRUST_FLAG_REFERENCE_WALK) && (p.flags & TRUST_FLAG_SHORT_WALK))
+        return fail(TRUST_ERR_ARG, "REFERENCE_WALK and SHORT_WALK together");
     return TRUST_OK;
 }
 
@@ -329,6 +335,88 @@ double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
     // offsets once + every N+(u) once + two offsets per walked v + every probe
     const double ob = off64 ? 8.0 : 4.0;
     return ob * (double)(V + 1) + 4.0 * (double)E + 2.0 * ob * (double)E + 4.0 * (double)W;
+}
+
+double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t I)
+{
+    // plus the in-walk offsets once, every in-walk entry, two offsets per entry
+    const double ob = off64 ? 8.0 : 4.0;
+    return alg_bytes(V, E, W, off64) + ob * (double)(V + 1) + 4.0 * (double)I +
+           2.0 * ob * (double)I;
+}
+
+// The walk of an unpartitioned count: the short side when forced or when
+// it makes at most kShortWalkShare of the reference walk's probes.  The
+// probe totals and the in-walk rows depend on the graph alone, so they are
+// made once per loaded graph, here, before any timed region, and the time
+// is reported as index_seconds.
+static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
+{
+    DevCsr &g = ctx->g;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool worked = false;
+    if (!force && !ctx->side_stats_ready) {
+        worked = true;
+        int rc = ctx->side_stats.ensure(4 * sizeof(unsigned long long));
+        if (rc)
+            return rc;
+        unsigned long long *st = (unsigned long long *)ctx->side_stats.p;
+        if (g.off64)
+            HIPCHK(launch_side_stats<int64_t>((const int64_t *)g.beg.p, (const int32_t *)g.adj.p,
+                                              g.V, st, ctx->stream));
+        else
+            HIPCHK(launch_side_stats<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
+                                              g.V, st, ctx->stream));
+        unsigned long long h[2];
+        HIPCHK(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->side_w_ref = h[0];
+        ctx->side_w_short = h[1];
+        ctx->side_stats_ready = true;
+    }
+    R.side = force || (double)ctx->side_w_short <= kShortWalkShare * (double)ctx->side_w_ref;
+    if (R.side && !ctx->side_built) {
+        worked = true;
+        const size_t ob = g.off64 ? sizeof(int64_t) : sizeof(int32_t);
+        int rc = ctx->side_cnt.ensure(sizeof(int) * (size_t)g.V);
+        if (!rc)
+            rc = ctx->side_bsum.ensure(sizeof(long long) * (size_t)(side_scan_blocks(g.V) + 1));
+        if (!rc)
+            rc = ctx->side_beg.ensure(ob * (size_t)(g.V + 1));
+        if (!rc)
+            rc = ctx->side_adj.ensure(sizeof(int32_t) * (size_t)g.E);
+        if (rc)
+            return rc;
+        if (g.off64)
+            HIPCHK(launch_side_build<int64_t>((const int64_t *)g.beg.p, (const int32_t *)g.adj.p,
+                                              g.V, (int *)ctx->side_cnt.p,
+                                              (long long *)ctx->side_bsum.p,
+                                              (int64_t *)ctx->side_beg.p,
+                                              (int32_t *)ctx->side_adj.p, ctx->stream));
+        else
+            HIPCHK(launch_side_build<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
+                                              g.V, (int *)ctx->side_cnt.p,
+                                              (long long *)ctx->side_bsum.p,
+                                              (int32_t *)ctx->side_beg.p,
+                                              (int32_t *)ctx->side_adj.p, ctx->stream));
+        if (g.off64) {
+            int64_t n = 0;
+            HIPCHK(hipMemcpyAsync(&n, (const int64_t *)ctx->side_beg.p + g.V, sizeof(n),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->side_in_entries = n;
+        } else {
+            int32_t n = 0;
+            HIPCHK(hipMemcpyAsync(&n, (const int32_t *)ctx->side_beg.p + g.V, sizeof(n),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->side_in_entries = n;
+        }
+        ctx->side_built = true;
+    }
+    R.index_seconds =
+        worked ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() : 0.0;
+    return TRUST_OK;
 }
 
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
@@ -350,6 +438,13 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
     rc = ctx->ctl.ensure(sizeof(TcCtl));
     if (rc)
         return rc;
+    R.side = false;
+    R.index_seconds = 0;
+    if (!(p.flags & TRUST_FLAG_REFERENCE_WALK) && g.V > 0 && g.E > 0) {
+        rc = prepare_side(ctx, (p.flags & TRUST_FLAG_SHORT_WALK) != 0, R);
+        if (rc)
+            return rc;
+    }
     if (mode == kModeInstr) {
         rc = ctx->instr_cycles.ensure(sizeof(unsigned long long) * (size_t)std::max<int64_t>(g.V, 1));
         if (!rc)
@@ -384,6 +479,10 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
     if (timed)
         HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     int rc;
+    if (R.side) {  // in-walk rows built by prepare_side
+        v.begI = ctx->side_beg.p;
+        v.adjI = (const int32_t *)ctx->side_adj.p;
+    }
     if (R.learned) {
         rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream);
         if (rc)
@@ -420,7 +519,8 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
     r->seconds = t_plan + t_count;
     r->h2d_seconds = ctx->h2d_seconds;
     r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
-    r->alg_bytes = alg_bytes(g.V, g.E, h.probes, g.off64);
+    r->alg_bytes = R.side ? alg_bytes_short(g.V, g.E, h.probes, g.off64, ctx->side_in_entries)
+                          : alg_bytes(g.V, g.E, h.probes, g.off64);
     r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
     r->n_hub = h.list_begin[kTierHub + 1] - h.list_begin[kTierBig];  // big + hub
     r->n_wave = h.list_begin[kTierWave + 1] - h.list_begin[kTierWave];
@@ -429,6 +529,8 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
     r->hub_slots = 1 << L.main.group_lg;
     r->wave_slots = L.main.wave_slots;
     r->schedule_used = R.learned ? TRUST_SCHED_LEARNED : TRUST_SCHED_DEGREE;
+    r->walk_used = R.side ? 1 : 0;
+    r->index_seconds = R.index_seconds;
 }
 
 // Count over the context's resident graph.
@@ -517,6 +619,7 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
     drop_captured(ctx);
     ctx->batch_ngraphs = 0;
     ctx->judge = -1;  // the judgement belongs to the graph; the model stays
+    ctx->side_stats_ready = ctx->side_built = false;  // the walk index belongs to the graph
     auto t0 = std::chrono::steady_clock::now();
     int rc = upload_csr(ctx, ctx->g, beg, adj, V, E, (flags & TRUST_FLAG_OFFSETS64) != 0,
                         ctx->stream);

@@ -272,6 +272,8 @@ extern "C" int trust_count_tiles(const trust_csr *tiles, int n, const trust_para
     int rc = check_params(*p);
     if (rc)
         return rc;
+    if (p->flags & TRUST_FLAG_SHORT_WALK)  // the in-walk rows of a tile job would span tiles
+        return fail(TRUST_ERR_ARG, "the short-side walk needs an unpartitioned graph");
     Ctx ctx;
     rc = ctx_init(&ctx, p->device);
     if (rc)

@@ -87,6 +87,13 @@ struct TcArgs {
     TcCtl *ctl;
     unsigned long long *instr_cycles;  // per vertex (instrumented kernel only)
     unsigned long long *instr_probes;
+    // Short-side walk (nullable, unpartitioned graphs only): row u of the
+    // walk is then the v in N+(u) with d+(v) <= d+(u), followed by the
+    // in-walk row adjI[begI[u], begI[u+1]) = the w with w -> u and
+    // 2 <= d+(w) < d+(u).  Every oriented edge is intersected once, from the
+    // side whose list is shorter.
+    const OffT *begI;
+    const int32_t *adjI;
 };
 
 // Planner parameters for the degree split.
@@ -104,6 +111,7 @@ struct PlanArgs {
     int force_spill;
     int wave_only;
     const unsigned char *tier_override;  // nullable: per-vertex tier chosen elsewhere
+    const OffT *begI;   // nullable: in-walk rows, counted into the walk length
     int32_t *list;
     long long *block_counts;  // [kNumTiers][nblocks]
     int per_block;
@@ -123,6 +131,25 @@ hipError_t launch_copy(const int4 *src, int4 *dst, long long n, int blocks, hipS
 hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
                                  hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream_t stream);
+
+// Builds the in-walk rows of the short-side walk for an oriented CSR:
+// adjI row v lists every u with u -> v, d+(u) >= 2 and d+(v) > d+(u).
+// cnt: V ints of scratch; bsum: side_scan_blocks(V) + 1 long longs;
+// begI: V + 1 offsets; adjI: up to E ids.  Launches only (no host sync).
+constexpr int kSideScanPer = 8192;  // vertices per scan workgroup
+inline long long side_scan_blocks(long long V) { return (V + kSideScanPer - 1) / kSideScanPer; }
+template <typename OffT>
+hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
+                             long long *bsum, OffT *begI, int32_t *adjI, hipStream_t stream);
+// stats[0] = probes of the reference walk, stats[1] = of the short-side walk
+// (stats: 2 unsigned long longs, cleared here).
+template <typename OffT>
+hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
+                             unsigned long long *stats, hipStream_t stream);
+// The short-side walk is chosen by default when it makes at most this share
+// of the reference walk's probes (its lists are shorter, so a probe costs
+// more; measured break-even near 0.45-0.5 on R-MAT).
+constexpr double kShortWalkShare = 0.45;
 
 // LDS ints besides the tables: per-wave marks, misc words, chunk totals.
 inline int lds_extra_words(int threads) { return (threads / kWave) * kWave + 8 + kMaxChunks + 8; }
