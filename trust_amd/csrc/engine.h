@@ -80,6 +80,9 @@ struct Ctx {
     bool ready = false;
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;
+    // the large-LDS launch runs here, beside the main launch (fork / join)
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int num_cus = 0;
     size_t lds_max = 0;
@@ -155,8 +158,9 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
 // Learned scheduler (learned.hip): enqueues features, MLP inference, tier
 // choice and the longest-first counting sort that fill ctx->list and
 // ctl->list_begin for the resident graph.
+// begI: the in-walk offsets when the count takes the short-side walk, else null.
 int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
-                         hipStream_t s);
+                         hipStream_t s, const void *begI);
 // Instrumented wave-only pass, GPU features, GPU training of the MLP.
 int train_cost_model(Ctx *ctx, const trust_params &p, int epochs, long long max_samples,
                      uint64_t seed, double *loss_out, double *seconds_out);

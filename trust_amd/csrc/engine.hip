@@ -281,19 +281,36 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
     if (ev_plan)
         HIPCHK(hipEventRecord(ev_plan, s));
 
+    // The large-LDS launch (big + spill tiers) and the main launch (hub +
+    // wave tiers) share nothing but the result counters, so the big one runs
+    // on the auxiliary stream and the main launch fills the CUs it frees.
+    // Under stream capture both stay on s.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(s, &cap));
+    const bool fork = L.has_big && cap == hipStreamCaptureStatusNone && ctx->aux_stream;
     TcArgs<OffT> a;
     if (L.has_big) {
         fill_args(a, ctx, views, p, L, L.big, ctl_dev, instr_cycles, instr_probes);
         a.spill_tier = L.spill_groups > 0 ? kTierSpill : -1;
         a.group_tier = kTierBig;
         a.wave_tier = -1;
-        HIPCHK(launch_count<OffT>(a, L.big.blocks, L.big.threads, L.big.lds_bytes, mode, true, s));
+        hipStream_t sb = s;
+        if (fork) {
+            HIPCHK(hipEventRecord(ctx->ev_fork, s));
+            HIPCHK(hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
+            sb = ctx->aux_stream;
+        }
+        HIPCHK(launch_count<OffT>(a, L.big.blocks, L.big.threads, L.big.lds_bytes, mode, true, sb));
+        if (fork)
+            HIPCHK(hipEventRecord(ctx->ev_join, ctx->aux_stream));
     }
     fill_args(a, ctx, views, p, L, L.main, ctl_dev, instr_cycles, instr_probes);
     a.spill_tier = -1;
     a.group_tier = kTierHub;
     a.wave_tier = kTierWave;
     HIPCHK(launch_count<OffT>(a, L.main.blocks, L.main.threads, L.main.lds_bytes, mode, false, s));
+    if (fork)
+        HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     if (ev_count)
         HIPCHK(hipEventRecord(ev_count, s));
     return TRUST_OK;
@@ -484,7 +501,7 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
         v.adjI = (const int32_t *)ctx->side_adj.p;
     }
     if (R.learned) {
-        rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream);
+        rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream, v.begI);
         if (rc)
             return rc;
     }
@@ -583,6 +600,9 @@ int ctx_init(Ctx *ctx, int device)
     ctx->lds_max = prop.sharedMemPerBlock;
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     for (auto &ev : ctx->ev)
         HIPCHK(hipEventCreate(&ev));
     ctx->ready = true;
@@ -596,13 +616,17 @@ Ctx::~Ctx()
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
     (void)hipStreamSynchronize(copy_stream);
+    (void)hipStreamSynchronize(aux_stream);
     drop_captured(this);
     g.beg.release();
     g.adj.release();
     for (auto &ev : this->ev)
         (void)hipEventDestroy(ev);
+    (void)hipEventDestroy(ev_fork);
+    (void)hipEventDestroy(ev_join);
     (void)hipStreamDestroy(stream);
     (void)hipStreamDestroy(copy_stream);
+    (void)hipStreamDestroy(aux_stream);
 }
 
 int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, int64_t E,

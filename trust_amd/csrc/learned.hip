@@ -213,12 +213,15 @@ struct KeyArgs {
     int same_csr;
     int wave_lg, hub_lg, big_lg;
     int force_spill;
+    int wave_only;
+    int hub_threshold;      // walk length above which a vertex takes a workgroup
     double split_fraction;  // wave -> workgroup when cost > fraction * total / waves
     double waves;
 };
 
 template <typename OffT>
-__global__ void __launch_bounds__(256) keys_kernel(const OffT *__restrict__ beg, KeyArgs k,
+__global__ void __launch_bounds__(256) keys_kernel(const OffT *__restrict__ beg,
+                                                   const OffT *__restrict__ begI, KeyArgs k,
                                                    const float *__restrict__ cost,
                                                    const double *__restrict__ total,
                                                    unsigned short *__restrict__ keys)
@@ -239,6 +242,9 @@ __global__ void __launch_bounds__(256) keys_kernel(const OffT *__restrict__ beg,
                 tier = kTierBig;
             else if (lg > k.wave_lg)
                 tier = kTierHub;
+            else if (!k.wave_only &&
+                     d + (begI ? (long long)(begI[u + 1] - begI[u]) : 0ll) > k.hub_threshold)
+                tier = kTierHub;  // long walks take a workgroup, as in the degree split
             else
                 tier = (double)cost[u] > tau ? kTierHub : kTierWave;
             int bin = (int)(log2f(1.f + cost[u]) * 2.f);
@@ -396,7 +402,7 @@ static int features_dispatch(Ctx *ctx, float *feat, hipStream_t s)
 
 template <typename OffT>
 static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
-                                  TcCtl *ctl, hipStream_t s)
+                                  TcCtl *ctl, hipStream_t s, const void *begI)
 {
     const long long V = ctx->g.V;
     int rc;
@@ -429,10 +435,12 @@ static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchP
     k.hub_lg = L.main.group_lg;
     k.big_lg = L.has_big ? L.big.group_lg : 0;
     k.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
+    k.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
+    k.hub_threshold = p.hub_threshold;
     k.split_fraction = 0.5;
     k.waves = (double)L.main.blocks * (L.main.threads / kWave);
-    hipLaunchKernelGGL(keys_kernel<OffT>, dim3(cb), dim3(256), 0, s, (const OffT *)ctx->g.beg.p, k,
-                       (const float *)ctx->cost.p, (const double *)total,
+    hipLaunchKernelGGL(keys_kernel<OffT>, dim3(cb), dim3(256), 0, s, (const OffT *)ctx->g.beg.p,
+                       (const OffT *)begI, k, (const float *)ctx->cost.p, (const double *)total,
                        (unsigned short *)ctx->tiers.p);
     hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, s,
                        (const unsigned short *)ctx->tiers.p, V, per_block, counts, nb);
@@ -447,12 +455,12 @@ static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchP
 }
 
 int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
-                         hipStream_t s)
+                         hipStream_t s, const void *begI)
 {
     if (!ctx->mlp_ready)
         return fail(TRUST_ERR_ARG, "no cost model");
-    return ctx->g.off64 ? enqueue_learned_plan_t<int64_t>(ctx, p, L, ctl, s)
-                        : enqueue_learned_plan_t<int32_t>(ctx, p, L, ctl, s);
+    return ctx->g.off64 ? enqueue_learned_plan_t<int64_t>(ctx, p, L, ctl, s, begI)
+                        : enqueue_learned_plan_t<int32_t>(ctx, p, L, ctl, s, begI);
 }
 
 // Instrumented wave-only pass -> features -> GPU training.
