@@ -48,9 +48,6 @@ struct CsrViews {
     const int32_t *adjW;
     const void *begP;
     const int32_t *adjP;
-    // in-walk rows of the short-side walk (null: the reference's walk)
-    const void *begI = nullptr;
-    const int32_t *adjI = nullptr;
 };
 
 // Shape of one counting launch.
@@ -90,12 +87,12 @@ struct Ctx {
     double h2d_seconds = 0;
     double learned_plan_seconds = 0;
     DevBuf staging, list, block_counts, ctl, spill, instr_cycles, instr_probes;
-    // short-side walk: probe totals of both walks and the in-walk rows, a
+    // short-side walk: probe totals of both walks and the walk rows, a
     // per-graph index made the first time a count needs it
     DevBuf side_cnt, side_bsum, side_beg, side_adj, side_stats;
     bool side_stats_ready = false, side_built = false;
     uint64_t side_w_ref = 0, side_w_short = 0;
-    int64_t side_in_entries = 0;  // entries of the in-walk rows
+    int64_t side_entries = 0;  // entries of the walk rows
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     bool mlp_ready = false;
@@ -124,8 +121,8 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
                    LaunchPlan &L, int mode = kModeCount);
 int check_params(const trust_params &p);
 double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64);
-// The same for the short-side walk, whose in-walk rows hold I entries.
-double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t I);
+// The same for the short-side walk, whose walk rows hold X entries.
+double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t X);
 
 template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
@@ -158,9 +155,10 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
 // Learned scheduler (learned.hip): enqueues features, MLP inference, tier
 // choice and the longest-first counting sort that fill ctx->list and
 // ctl->list_begin for the resident graph.
-// begI: the in-walk offsets when the count takes the short-side walk, else null.
+// begWalk: the walk offsets when the count takes the short-side walk, else
+// null (the walk is N+(u)).
 int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
-                         hipStream_t s, const void *begI);
+                         hipStream_t s, const void *begWalk);
 // Instrumented wave-only pass, GPU features, GPU training of the MLP.
 int train_cost_model(Ctx *ctx, const trust_params &p, int epochs, long long max_samples,
                      uint64_t seed, double *loss_out, double *seconds_out);

@@ -222,8 +222,6 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.ctl = ctl_dev;
     a.instr_cycles = instr_cycles;
     a.instr_probes = instr_probes;
-    a.begI = (const OffT *)views.begI;
-    a.adjI = views.adjI;
     if (ctx->batch_ngraphs > 0) {  // read by the batch mode only
         const int64_t ng = ctx->batch_ngraphs;
         a.graph_vbase = (const long long *)ctx->batch_vbase.p;
@@ -271,7 +269,6 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
         pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
         pa.tier_override = nullptr;
-        pa.begI = (const OffT *)views.begI;
         pa.list = (int32_t *)ctx->list.p;
         pa.block_counts = (long long *)ctx->block_counts.p;
         pa.per_block = per_block;
@@ -354,12 +351,13 @@ double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
     return ob * (double)(V + 1) + 4.0 * (double)E + 2.0 * ob * (double)E + 4.0 * (double)W;
 }
 
-double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t I)
+double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t X)
 {
-    // plus the in-walk offsets once, every in-walk entry, two offsets per entry
+    // hash rows: offsets once + every N+(u); walk rows: offsets once + every
+    // entry; two offsets per walked entry; every probe
     const double ob = off64 ? 8.0 : 4.0;
-    return alg_bytes(V, E, W, off64) + ob * (double)(V + 1) + 4.0 * (double)I +
-           2.0 * ob * (double)I;
+    return 2.0 * ob * (double)(V + 1) + 4.0 * (double)E + 4.0 * (double)X +
+           2.0 * ob * (double)X + 4.0 * (double)W;
 }
 
 // The walk of an unpartitioned count: the short side when forced or when
@@ -395,7 +393,7 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
     if (R.side && !ctx->side_built) {
         worked = true;
         const size_t ob = g.off64 ? sizeof(int64_t) : sizeof(int32_t);
-        int rc = ctx->side_cnt.ensure(sizeof(int) * (size_t)g.V);
+        int rc = ctx->side_cnt.ensure(2 * sizeof(int) * (size_t)g.V);
         if (!rc)
             rc = ctx->side_bsum.ensure(sizeof(long long) * (size_t)(side_scan_blocks(g.V) + 1));
         if (!rc)
@@ -421,13 +419,13 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
             HIPCHK(hipMemcpyAsync(&n, (const int64_t *)ctx->side_beg.p + g.V, sizeof(n),
                                   hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
-            ctx->side_in_entries = n;
+            ctx->side_entries = n;
         } else {
             int32_t n = 0;
             HIPCHK(hipMemcpyAsync(&n, (const int32_t *)ctx->side_beg.p + g.V, sizeof(n),
                                   hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
-            ctx->side_in_entries = n;
+            ctx->side_entries = n;
         }
         ctx->side_built = true;
     }
@@ -496,12 +494,13 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
     if (timed)
         HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     int rc;
-    if (R.side) {  // in-walk rows built by prepare_side
-        v.begI = ctx->side_beg.p;
-        v.adjI = (const int32_t *)ctx->side_adj.p;
+    if (R.side) {  // walk rows built by prepare_side; hash and probe rows stay N+
+        v.begW = ctx->side_beg.p;
+        v.adjW = (const int32_t *)ctx->side_adj.p;
     }
     if (R.learned) {
-        rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream, v.begI);
+        rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream,
+                                  R.side ? v.begW : nullptr);
         if (rc)
             return rc;
     }
@@ -536,7 +535,7 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
     r->seconds = t_plan + t_count;
     r->h2d_seconds = ctx->h2d_seconds;
     r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
-    r->alg_bytes = R.side ? alg_bytes_short(g.V, g.E, h.probes, g.off64, ctx->side_in_entries)
+    r->alg_bytes = R.side ? alg_bytes_short(g.V, g.E, h.probes, g.off64, ctx->side_entries)
                           : alg_bytes(g.V, g.E, h.probes, g.off64);
     r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
     r->n_hub = h.list_begin[kTierHub + 1] - h.list_begin[kTierBig];  // big + hub

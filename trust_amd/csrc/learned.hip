@@ -221,7 +221,7 @@ struct KeyArgs {
 
 template <typename OffT>
 __global__ void __launch_bounds__(256) keys_kernel(const OffT *__restrict__ beg,
-                                                   const OffT *__restrict__ begI, KeyArgs k,
+                                                   const OffT *__restrict__ begWalk, KeyArgs k,
                                                    const float *__restrict__ cost,
                                                    const double *__restrict__ total,
                                                    unsigned short *__restrict__ keys)
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(256) keys_kernel(const OffT *__restrict__ beg,
             else if (lg > k.wave_lg)
                 tier = kTierHub;
             else if (!k.wave_only &&
-                     d + (begI ? (long long)(begI[u + 1] - begI[u]) : 0ll) > k.hub_threshold)
+                     (begWalk ? (long long)(begWalk[u + 1] - begWalk[u]) : d) > k.hub_threshold)
                 tier = kTierHub;  // long walks take a workgroup, as in the degree split
             else
                 tier = (double)cost[u] > tau ? kTierHub : kTierWave;
@@ -402,7 +402,7 @@ static int features_dispatch(Ctx *ctx, float *feat, hipStream_t s)
 
 template <typename OffT>
 static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
-                                  TcCtl *ctl, hipStream_t s, const void *begI)
+                                  TcCtl *ctl, hipStream_t s, const void *begWalk)
 {
     const long long V = ctx->g.V;
     int rc;
@@ -440,7 +440,7 @@ static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchP
     k.split_fraction = 0.5;
     k.waves = (double)L.main.blocks * (L.main.threads / kWave);
     hipLaunchKernelGGL(keys_kernel<OffT>, dim3(cb), dim3(256), 0, s, (const OffT *)ctx->g.beg.p,
-                       (const OffT *)begI, k, (const float *)ctx->cost.p, (const double *)total,
+                       (const OffT *)begWalk, k, (const float *)ctx->cost.p, (const double *)total,
                        (unsigned short *)ctx->tiers.p);
     hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, s,
                        (const unsigned short *)ctx->tiers.p, V, per_block, counts, nb);
@@ -455,12 +455,12 @@ static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchP
 }
 
 int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
-                         hipStream_t s, const void *begI)
+                         hipStream_t s, const void *begWalk)
 {
     if (!ctx->mlp_ready)
         return fail(TRUST_ERR_ARG, "no cost model");
-    return ctx->g.off64 ? enqueue_learned_plan_t<int64_t>(ctx, p, L, ctl, s, begI)
-                        : enqueue_learned_plan_t<int32_t>(ctx, p, L, ctl, s, begI);
+    return ctx->g.off64 ? enqueue_learned_plan_t<int64_t>(ctx, p, L, ctl, s, begWalk)
+                        : enqueue_learned_plan_t<int32_t>(ctx, p, L, ctl, s, begWalk);
 }
 
 // Instrumented wave-only pass -> features -> GPU training.

@@ -87,13 +87,6 @@ struct TcArgs {
     TcCtl *ctl;
     unsigned long long *instr_cycles;  // per vertex (instrumented kernel only)
     unsigned long long *instr_probes;
-    // Short-side walk (nullable, unpartitioned graphs only): row u of the
-    // walk is then the v in N+(u) with d+(v) <= d+(u), followed by the
-    // in-walk row adjI[begI[u], begI[u+1]) = the w with w -> u and
-    // 2 <= d+(w) < d+(u).  Every oriented edge is intersected once, from the
-    // side whose list is shorter.
-    const OffT *begI;
-    const int32_t *adjI;
 };
 
 // Planner parameters for the degree split.
@@ -111,7 +104,6 @@ struct PlanArgs {
     int force_spill;
     int wave_only;
     const unsigned char *tier_override;  // nullable: per-vertex tier chosen elsewhere
-    const OffT *begI;   // nullable: in-walk rows, counted into the walk length
     int32_t *list;
     long long *block_counts;  // [kNumTiers][nblocks]
     int per_block;
@@ -132,15 +124,17 @@ hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
                                  hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream_t stream);
 
-// Builds the in-walk rows of the short-side walk for an oriented CSR:
-// adjI row v lists every u with u -> v, d+(u) >= 2 and d+(v) > d+(u).
-// cnt: V ints of scratch; bsum: side_scan_blocks(V) + 1 long longs;
-// begI: V + 1 offsets; adjI: up to E ids.  Launches only (no host sync).
+// Builds the walk rows of the short-side walk for an oriented CSR (the walk
+// view of the three-view kernel): row x = the v in N+(x) with
+// d+(v) <= d+(x), then the u with u -> x and d+(u) < d+(x), rows with
+// d+ < 2 contributing nothing.  cnt: 2V ints of scratch; bsum:
+// side_scan_blocks(V) + 1 long longs; begX: V + 1 offsets; adjX: up to E
+// ids.  Launches only (no host sync).
 constexpr int kSideScanPer = 8192;  // vertices per scan workgroup
 inline long long side_scan_blocks(long long V) { return (V + kSideScanPer - 1) / kSideScanPer; }
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begI, int32_t *adjI, hipStream_t stream);
+                             long long *bsum, OffT *begX, int32_t *adjX, hipStream_t stream);
 // stats[0] = probes of the reference walk, stats[1] = of the short-side walk
 // (stats: 2 unsigned long longs, cleared here).
 template <typename OffT>

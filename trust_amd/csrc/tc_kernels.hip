@@ -340,18 +340,14 @@ struct Group {
     int total;  // uniform: sum of the 64 degrees
 };
 
-// A list longer than `cap` is dropped (length 0): the short-side walk's
-// out-part keeps v only when d+(v) <= d+(u).
 template <typename OffT>
-__device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT *__restrict__ begP,
-                                                  int cap)
+__device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT *__restrict__ begP)
 {
     OffT s = 0;
     int deg = 0;
     if (valid) {
         s = begP[v];
         deg = (int)(begP[v + 1] - s);
-        deg = deg <= cap ? deg : 0;
     }
     Group<OffT> g;
     g.deg = deg;
@@ -360,37 +356,6 @@ __device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT 
     g.total = readlane(g.incl, 63);
     g.base = s - (OffT)g.excl;
     return g;
-}
-
-// Vertex u's walk: positions [0, dW) are row u of adjW (lists longer than
-// `cap` dropped), positions [dW, dT) the in-walk row at adjI + sI.  Group of
-// the 64 positions starting at idx - lane.
-template <typename OffT>
-__device__ __forceinline__ Group<OffT> walk_group(const TcArgs<OffT> &a, OffT sW, int dW, OffT sI,
-                                                  int dT, int cap, int idx)
-{
-    int v = 0;
-    if (idx < dW)
-        v = a.adjW[sW + (OffT)idx];
-    else if (idx < dT)
-        v = a.adjI[sI + (OffT)(idx - dW)];
-    return load_group(v, idx < dT, a.begP, idx < dW ? cap : INT_MAX);
-}
-
-// Bounds of u's in-walk row (none without the short-side walk) and the
-// out-part cap.
-template <typename OffT>
-__device__ __forceinline__ void in_walk(const TcArgs<OffT> &a, int u, int dH, OffT &sI, int &dI,
-                                       int &cap)
-{
-    sI = 0;
-    dI = 0;
-    cap = INT_MAX;
-    if (a.begI) {
-        sI = a.begI[u];
-        dI = (int)(a.begI[u + 1] - sI);
-        cap = dH;
-    }
 }
 
 // Probes windows [wbeg, wend) (64 positions each) of a group's
@@ -547,10 +512,6 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
     const int dH = (int)(a.begH[u + 1] - sH);
     const OffT sW = a.begW[u];
     const int dW = (int)(a.endW[u] - sW);
-    OffT sI;
-    int dI, cap;
-    in_walk(a, u, dH, sI, dI, cap);
-    const int dT = dW + dI;
     const int lg = table_lg(dH, cap_lg);
     const int size = 1 << lg;
     if constexpr (kTable)
@@ -589,13 +550,15 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
     if constexpr (kGlobal)
         __threadfence();
     __syncthreads();
-    const int nchunks = (dT + kWave - 1) / kWave;
+    const int nchunks = (dW + kWave - 1) / kWave;
     unsigned long long hits = 0;  // wave-uniform
     unsigned long long wave_probes = 0;
     if (nchunks <= kMaxChunks) {
         // window count of every 64-neighbour chunk, then an exclusive prefix
         for (int c = wave; c < nchunks; c += nwaves) {
-            const Group<OffT> g = walk_group(a, sW, dW, sI, dT, cap, c * kWave + lane);
+            const int idx = c * kWave + lane;
+            const bool valid = idx < dW;
+            const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
             if (lane == 0)
                 ctot[c] = (g.total + kWave - 1) / kWave;
             wave_probes += (unsigned long long)g.total;
@@ -642,7 +605,9 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
                         hi = mid;
                 }
                 if (lo != cur) {
-                    g = walk_group(a, sW, dW, sI, dT, cap, lo * kWave + lane);
+                    const int idx = lo * kWave + lane;
+                    const bool valid = idx < dW;
+                    g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
                     cur = lo;
                 }
                 const int cend = min(w1, ctot[lo + 1]);
@@ -660,7 +625,9 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
             c = __builtin_amdgcn_readfirstlane(c);
             if (c >= nchunks)
                 break;
-            const Group<OffT> g = walk_group(a, sW, dW, sI, dT, cap, c * kWave + lane);
+            const int idx = c * kWave + lane;
+            const bool valid = idx < dW;
+            const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
             wave_probes += (unsigned long long)g.total;
             hits += probe_windows<OffT, kGlobal, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab,
                                                  lg, linear, mark);
@@ -695,10 +662,6 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     const int dH = (int)(a.begH[u + 1] - sH);
     const OffT sW = a.begW[u];
     const int dW = (int)(a.endW[u] - sW);
-    OffT sI;
-    int dI, cap;
-    in_walk(a, u, dH, sI, dI, cap);
-    const int dT = dW + dI;
     const int lg = table_lg(dH, a.wave_lg);
     const int size = 1 << lg;
     // On an unpartitioned graph the hashed row and the walked row are one
@@ -733,14 +696,10 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     }
     unsigned long long hits = 0;  // wave-uniform
     unsigned long long wave_probes = 0;
-    for (int c = 0; c < dT; c += kWave) {
-        const int idx = c + lane;
-        int v = 0;
-        if (idx < dW)
-            v = c == 0 && same_row ? head : a.adjW[sW + (OffT)idx];
-        else if (idx < dT)
-            v = a.adjI[sI + (OffT)(idx - dW)];
-        const Group<OffT> g = load_group(v, idx < dT, a.begP, idx < dW ? cap : INT_MAX);
+    for (int c = 0; c < dW; c += kWave) {
+        const bool valid = c + lane < dW;
+        const int v = c == 0 && same_row ? head : (valid ? a.adjW[sW + (OffT)(c + lane)] : 0);
+        const Group<OffT> g = load_group(v, valid, a.begP);
         wave_probes += (unsigned long long)g.total;
         hits += probe_windows<OffT, false, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab, lg,
                                            linear, mark);
@@ -865,8 +824,7 @@ __device__ __forceinline__ int plan_tier(const PlanArgs<OffT> &p, long long u)
     if (p.tier_override)
         return (int)p.tier_override[u];
     const long long dH = (long long)(p.begH[u + 1] - p.begH[u]);
-    const long long dW = (long long)(p.endW[u] - p.begW[u]) +
-                         (p.begI ? (long long)(p.begI[u + 1] - p.begI[u]) : 0ll);
+    const long long dW = (long long)(p.endW[u] - p.begW[u]);
     if (dH == 0 || dW == 0 || (p.same_csr && dH < 2))
         return kTierNone;  // contributes nothing (reference skips d+ < 2, TC.cu:440-441)
     int lg = kMinTableLg;
@@ -993,28 +951,38 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
 }
 
 // ------------------------------------------------------ short-side walk
-// The in-walk rows.  A wave takes 64 consecutive rows u, whose lists are
-// contiguous in adj, and walks them in coalesced 64-wide windows; the row of
-// each position comes from a mark array and a max-scan.  An edge u -> v goes
-// to row v when d+(u) >= 2 and d+(v) > d+(u) (an edge from a vertex with
-// d+ < 2 closes no triangle at that edge).  kPass 0 sums the probes of the
-// reference walk and of the short-side walk into stats[0], stats[1]; kPass 1
-// counts each row's entries into cnt; kPass 2 hands out slots by counting
-// cnt back down, so cnt is all zero again afterwards.
+// The walk rows of the short-side walk.  Row x lists first the v in N+(x)
+// with d+(v) <= d+(x) (N+(v) is probed against x's table), then the u with
+// u -> x and d+(u) < d+(x) (N+(u) is probed against x's table); an edge from
+// a vertex with d+ < 2 closes no triangle at that edge and is left out.
+// Every oriented edge that can close a triangle lands in exactly one row.
+//
+// A wave takes 64 consecutive rows u, whose lists are contiguous in adj, and
+// walks them in coalesced 64-wide windows; the row of each position comes
+// from a mark array and a max-scan.
+//   kPass 0: sums the probes of the reference walk and of the short-side
+//            walk into stats[0], stats[1];
+//   kPass 1: kept[u] = number of kept out-entries of row u (counted in LDS,
+//            each row belongs to one wave), incnt[v] += 1 per in-entry;
+//   kPass 2: writes the rows: out-entries at begX[u] + (LDS slot), in-entries
+//            from the row's end down (incnt counts back to zero).
 constexpr int kSideWin = 4;
 
 template <typename OffT, int kPass>
 __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__ beg,
                                                         const int32_t *__restrict__ adj,
-                                                        long long V, int *__restrict__ cnt,
-                                                        const OffT *__restrict__ begI,
-                                                        int32_t *__restrict__ adjI,
+                                                        long long V, int *__restrict__ incnt,
+                                                        int *__restrict__ kept,
+                                                        const OffT *__restrict__ begX,
+                                                        int32_t *__restrict__ adjX,
                                                         unsigned long long *__restrict__ stats)
 {
     unsigned long long w_ref = 0, w_short = 0;
     __shared__ int mark_s[4][kWave * kSideWin];
+    __shared__ int rowcnt_s[4][kWave];
     const int wave = threadIdx.x / kWave, lane = lane_id();
     int *mark = mark_s[wave];
+    int *rowcnt = rowcnt_s[wave];
     for (int k = 0; k < kSideWin; k++)
         mark[k * kWave + lane] = 0;
     const long long step = (long long)gridDim.x * (blockDim.x / kWave) * kWave;
@@ -1027,6 +995,8 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
         const long long e0 = (long long)readlane(s, 0);
         const long long e1 = n == kWave ? (long long)readlane(s, 63) + readlane(d, 63)
                                         : (long long)readlane(s, n);
+        rowcnt[lane] = 0;
+        wave_sync();
         int carry = 0;
         for (long long p = e0; p < e1; p += kWave * kSideWin) {
             // a non-empty row starting inside this span marks its position
@@ -1053,18 +1023,30 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                     if constexpr (kPass == 0) {
                         w_ref += (unsigned long long)dv;
                         w_short += (unsigned long long)min(du, dv);
-                    } else if (dv > du) {
-                        if constexpr (kPass == 2) {
-                            const int slot = atomicSub(cnt + v[k], 1) - 1;
-                            adjI[begI[v[k]] + (OffT)slot] = (int32_t)(u0 + own[k]);
+                    } else if constexpr (kPass == 1) {
+                        if (dv <= du)
+                            atomicAdd(rowcnt + own[k], 1);
+                        else
+                            atomicAdd(incnt + v[k], 1);
+                    } else {
+                        const long long u = u0 + own[k];
+                        if (dv <= du) {
+                            const int slot = atomicAdd(rowcnt + own[k], 1);
+                            adjX[begX[u] + (OffT)slot] = v[k];
                         } else {
-                            atomicAdd(cnt + v[k], 1);
+                            const int k2 = atomicSub(incnt + v[k], 1);
+                            adjX[begX[v[k] + 1] - (OffT)k2] = (int32_t)u;
                         }
                     }
                 }
             }
             wave_sync();
         }
+        if constexpr (kPass == 1) {
+            if (lane < n)
+                kept[u0 + lane] = rowcnt[lane];
+        }
+        wave_sync();
     }
     if constexpr (kPass == 0) {
         w_ref = wave_sum(w_ref);
@@ -1076,9 +1058,10 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
     }
 }
 
-// Exclusive scan of cnt[0, V) into begI[0, V] in three launches: per-block
-// sums, one workgroup scanning those, then per-block scans.
-__global__ void __launch_bounds__(1024) side_scan_reduce_kernel(const int *__restrict__ cnt,
+// Exclusive scan of kept[i] + incnt[i], i < V, into begX[0, V] in three
+// launches: per-block sums, one workgroup scanning those, per-block scans.
+__global__ void __launch_bounds__(1024) side_scan_reduce_kernel(const int *__restrict__ incnt,
+                                                                const int *__restrict__ kept,
                                                                 long long V,
                                                                 long long *__restrict__ bsum)
 {
@@ -1087,7 +1070,7 @@ __global__ void __launch_bounds__(1024) side_scan_reduce_kernel(const int *__res
     const long long hi = min(lo + kSideScanPer, V);
     long long x = 0;
     for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x)
-        x += cnt[i];
+        x += (long long)incnt[i] + kept[i];
     x = (long long)wave_sum((unsigned long long)x);
     if (lane_id() == 0)
         part[threadIdx.x / kWave] = x;
@@ -1103,7 +1086,7 @@ __global__ void __launch_bounds__(1024) side_scan_reduce_kernel(const int *__res
 template <typename OffT>
 __global__ void __launch_bounds__(1024) side_scan_top_kernel(long long *__restrict__ bsum,
                                                              long long nb, long long V,
-                                                             OffT *__restrict__ begI)
+                                                             OffT *__restrict__ begX)
 {
     __shared__ long long part[1024];
     __shared__ long long base;
@@ -1129,14 +1112,15 @@ __global__ void __launch_bounds__(1024) side_scan_top_kernel(long long *__restri
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        begI[V] = (OffT)base;
+        begX[V] = (OffT)base;
 }
 
 template <typename OffT>
-__global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restrict__ cnt,
+__global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restrict__ incnt,
+                                                             const int *__restrict__ kept,
                                                              long long V,
                                                              const long long *__restrict__ bsum,
-                                                             OffT *__restrict__ begI)
+                                                             OffT *__restrict__ begX)
 {
     // thread t scans elements [lo + 8t, lo + 8t + 8)
     constexpr int kPer = kSideScanPer / 1024;
@@ -1146,7 +1130,7 @@ __global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restri
     long long t = 0;
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-        x[k] = lo + k < V ? cnt[lo + k] : 0;
+        x[k] = lo + k < V ? incnt[lo + k] + kept[lo + k] : 0;
         t += x[k];
     }
     part[threadIdx.x] = t;
@@ -1161,35 +1145,42 @@ __global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restri
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
         if (lo + k < V)
-            begI[lo + k] = (OffT)run;
+            begX[lo + k] = (OffT)run;
         run += x[k];
     }
 }
 
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begI, int32_t *adjI, hipStream_t stream)
+                             long long *bsum, OffT *begX, int32_t *adjX, hipStream_t stream)
 {
     if (V <= 0)
         return hipSuccess;
+    int *incnt = cnt, *kept = cnt + V;
     const long long nb = side_scan_blocks(V);
     const int pass_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
-    hipError_t e = launch_fill_u32((uint32_t *)cnt, 0u, V, stream);
+    hipError_t e = launch_fill_u32((uint32_t *)incnt, 0u, V, stream);
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL((side_pass_kernel<OffT, 1>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, cnt, (const OffT *)nullptr, (int32_t *)nullptr,
+                       beg, adj, V, incnt, kept, (const OffT *)nullptr, (int32_t *)nullptr,
                        (unsigned long long *)nullptr);
-    hipLaunchKernelGGL(side_scan_reduce_kernel, dim3((unsigned)nb), dim3(1024), 0, stream, cnt, V,
-                       bsum);
+    hipLaunchKernelGGL(side_scan_reduce_kernel, dim3((unsigned)nb), dim3(1024), 0, stream,
+                       (const int *)incnt, (const int *)kept, V, bsum);
     hipLaunchKernelGGL(side_scan_top_kernel<OffT>, dim3(1), dim3(1024), 0, stream, bsum, nb, V,
-                       begI);
-    hipLaunchKernelGGL(side_scan_out_kernel<OffT>, dim3((unsigned)nb), dim3(1024), 0, stream, cnt,
-                       V, (const long long *)bsum, begI);
+                       begX);
+    hipLaunchKernelGGL(side_scan_out_kernel<OffT>, dim3((unsigned)nb), dim3(1024), 0, stream,
+                       (const int *)incnt, (const int *)kept, V, (const long long *)bsum, begX);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 2>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, cnt, (const OffT *)begI, adjI, (unsigned long long *)nullptr);
+                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, adjX,
+                       (unsigned long long *)nullptr);
     return hipGetLastError();
 }
+
+template hipError_t launch_side_build<int32_t>(const int32_t *, const int32_t *, long long, int *,
+                                               long long *, int32_t *, int32_t *, hipStream_t);
+template hipError_t launch_side_build<int64_t>(const int64_t *, const int32_t *, long long, int *,
+                                               long long *, int64_t *, int32_t *, hipStream_t);
 
 template <typename OffT>
 hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
@@ -1200,8 +1191,8 @@ hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
         return e;
     const int pass_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 0>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, (int *)nullptr, (const OffT *)nullptr, (int32_t *)nullptr,
-                       stats);
+                       beg, adj, V, (int *)nullptr, (int *)nullptr, (const OffT *)nullptr,
+                       (int32_t *)nullptr, stats);
     return hipGetLastError();
 }
 
@@ -1209,11 +1200,6 @@ template hipError_t launch_side_stats<int32_t>(const int32_t *, const int32_t *,
                                                unsigned long long *, hipStream_t);
 template hipError_t launch_side_stats<int64_t>(const int64_t *, const int32_t *, long long,
                                                unsigned long long *, hipStream_t);
-
-template hipError_t launch_side_build<int32_t>(const int32_t *, const int32_t *, long long, int *,
-                                               long long *, int32_t *, int32_t *, hipStream_t);
-template hipError_t launch_side_build<int64_t>(const int64_t *, const int32_t *, long long, int *,
-                                               long long *, int64_t *, int32_t *, hipStream_t);
 
 // Plain streaming copy: the known-traffic kernel that calibrates HBM byte
 // counters and gives the sequential-bandwidth reference.
