@@ -198,6 +198,10 @@ int trust_mlp_train(int device, int hidden, float *params, const float *x, const
                     int64_t n, int epochs, int batch, float lr, uint64_t seed, double *loss_out);
 /* Scaled features of every vertex of the resident graph, feat[V][3]. */
 int trust_ctx_features(trust_ctx *ctx, float *feat);
+/* The same features over the short-side walk's rows (d+(u), sum and max of
+ * the list lengths u's walk probes): what the learned plan uses when a count
+ * takes that walk.  Builds the walk index first if the context has none. */
+int trust_ctx_walk_features(trust_ctx *ctx, float *feat);
 /*
  * Trains the context's cost model: an instrumented wave-only count records
  * per-vertex cycles, then the MLP is trained on a seeded sample of at most

@@ -15,6 +15,7 @@ wzbxpy/TRUST root):
   probes_short      probes of the product's short-side walk (no reference
                     counterpart; min(d+(u), d+(v)) per oriented edge)
   features          per-vertex d+(u), sum d+(v), max d+(v)
+  features_short    the same over the short-side walk's rows
   mlp_*             numpy float64 forward/backward of the cost model
                     (no reference counterpart)
 Pinning: tests/test_oracle.py checks every function against closed forms,
@@ -54,6 +55,8 @@ def lib():
         h.oracle_probes.argtypes = [_P64, _P32, C.c_int64, _P64]
         h.oracle_probes_short.restype = C.c_uint64
         h.oracle_probes_short.argtypes = [_P64, _P32, C.c_int64, _P64]
+        h.oracle_features_short.restype = None
+        h.oracle_features_short.argtypes = [_P64, _P32, C.c_int64, C.POINTER(C.c_double)]
         h.oracle_features.restype = None
         h.oracle_features.argtypes = [_P64, _P32, C.c_int64, C.POINTER(C.c_double)]
         h.oracle_count_hash_sample.restype = C.c_uint64
@@ -224,6 +227,20 @@ def mlp_loss_grad(p, x, t, H=MLP_H):
     gb1 = dh1.sum(0)
     g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3, [gb3]])
     return g, float((err ** 2).mean())
+
+
+def features_short(beg, adj) -> np.ndarray:
+    beg, pb = _i64(beg)
+    adj, pa = _i32(adj)
+    V = len(beg) - 1
+    out = np.zeros((V, 3), np.float64)
+    lib().oracle_features_short(pb, pa, V, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def mlp_scaled_features_short(beg, adj):
+    """log2(1 + (d+, sum, max)) / 32 over the short-side walk's rows."""
+    return (np.log2(1.0 + features_short(beg, adj)) / 32.0).astype(np.float64)
 
 
 def mlp_scaled_features(beg, adj):

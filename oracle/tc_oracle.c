@@ -40,6 +40,9 @@
  *       reference counterpart; the count is the reference's): each oriented
  *       edge u -> v with d+(u) >= 2 costs min(d+(u), d+(v)) probes, charged
  *       to u when d+(v) <= d+(u) and to v otherwise.
+ *   oracle_features_short  the same features over the short-side walk's
+ *       rows (no reference counterpart): sum and max of min(d+(u), d+(v)),
+ *       charged as oracle_probes_short charges them.
  *   oracle_features     the learned scheduler's per-vertex features
  *       d+(u), sum_{v in N+(u)} d+(v), max_{v in N+(u)} d+(v).
  *   oracle_count_hash_batch  oracle_count_hash per graph of a batch (graph
@@ -364,6 +367,28 @@ void oracle_features(const int64_t *beg, const int32_t *adj, int64_t V, double *
         out[3 * u + 0] = (double)d;
         out[3 * u + 1] = s;
         out[3 * u + 2] = mx;
+    }
+}
+
+void oracle_features_short(const int64_t *beg, const int32_t *adj, int64_t V, double *out)
+{
+    for (int64_t u = 0; u < V; u++) {
+        out[3 * u + 0] = (double)(beg[u + 1] - beg[u]);
+        out[3 * u + 1] = 0.0;
+        out[3 * u + 2] = 0.0;
+    }
+    for (int64_t u = 0; u < V; u++) {
+        const int64_t du = beg[u + 1] - beg[u];
+        if (du < ORACLE_MIN_DEGREE)
+            continue;
+        for (int64_t e = beg[u]; e < beg[u + 1]; e++) {
+            const int64_t v = adj[e], dv = beg[v + 1] - beg[v];
+            const int64_t owner = dv <= du ? u : v;
+            const double len = (double)(dv <= du ? dv : du);
+            out[3 * owner + 1] += len;
+            if (len > out[3 * owner + 2])
+                out[3 * owner + 2] = len;
+        }
     }
 }
 

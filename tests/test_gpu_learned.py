@@ -97,6 +97,16 @@ def test_features_edge_cases(gpu, case):
     assert np.allclose(f, oracle.mlp_scaled_features(beg, adj), rtol=1e-5, atol=1e-6), case
 
 
+@pytest.mark.parametrize("scale,ef,seed", [(14, 16, 1), (16, 16, 1), (12, 32, 4)])
+def test_walk_features_match_oracle(gpu, scale, ef, seed):
+    # the flat segmented kernel over the walk index's per-entry rows and d+
+    beg, adj = T.rmat_csr(scale, ef, seed=seed)
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        f = c.walk_features()
+    assert np.allclose(f, oracle.mlp_scaled_features_short(beg, adj), rtol=1e-5, atol=1e-6)
+
+
 def test_learned_without_model_falls_back(gpu, rmat16):
     beg, adj, want = rmat16
     with T.Context(gpu) as c:

@@ -126,6 +126,25 @@ def test_probes_short_against_explicit_walk(seed):
     assert w < oracle.probes(beg, adj)  # R-MAT: strictly fewer probes than the reference walk
 
 
+@pytest.mark.parametrize("seed", range(2))
+def test_features_short_against_explicit_walk(seed):
+    # sum = the vertex's short-side probes; max from the explicit walk rows
+    beg, adj = T.rmat_csr(9, 8, seed=seed + 30)
+    f = oracle.features_short(beg, adj)
+    _, per = oracle.probes_short(beg, adj, per_vertex=True)
+    assert np.array_equal(f[:, 1], per.astype(np.float64))
+    d = np.diff(beg)
+    assert np.array_equal(f[:, 0], d.astype(np.float64))
+    ins = [[] for _ in range(len(d))]
+    for u in range(len(d)):
+        for v in adj[beg[u]:beg[u + 1]]:
+            ins[v].append(u)
+    for o in range(len(d)):
+        walk = ([d[v] for v in adj[beg[o]:beg[o + 1]] if d[v] <= d[o]] if d[o] >= 2 else [])
+        walk += [d[w] for w in ins[o] if 2 <= d[w] < d[o]]
+        assert f[o, 2] == (max(walk) if walk else 0), o
+
+
 def test_short_walk_tie_rule_mistake_is_caught():
     beg, adj = T.rmat_csr(9, 8, seed=21)
     bad, _ = _explicit_short_walk(beg, adj, in_ties=True)

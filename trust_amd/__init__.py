@@ -114,6 +114,7 @@ SIGNATURES = {
     "trust_mlp_train": (C.c_int, [C.c_int, C.c_int, _PF, _PF, _PF, C.c_int64, C.c_int, C.c_int,
                                   C.c_float, C.c_uint64, C.POINTER(C.c_double)]),
     "trust_ctx_features": (C.c_int, [C.c_void_p, _PF]),
+    "trust_ctx_walk_features": (C.c_int, [C.c_void_p, _PF]),
     "trust_ctx_train_cost_model": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, C.c_int64,
                                              C.c_uint64, C.POINTER(C.c_double),
                                              C.POINTER(C.c_double)]),
@@ -482,6 +483,12 @@ class Context:
         """GPU features, shape (V, 3): log2(1 + (d+, sum d+(v), max d+(v))) / 32."""
         out = np.zeros((self.V, 3), np.float32)
         _check(lib().trust_ctx_features(self._h, out.ctypes.data_as(_PF)))
+        return out
+
+    def walk_features(self) -> np.ndarray:
+        """GPU features over the short-side walk's rows, shape (V, 3)."""
+        out = np.zeros((self.V, 3), np.float32)
+        _check(lib().trust_ctx_walk_features(self._h, out.ctypes.data_as(_PF)))
         return out
 
     def train_cost_model(self, epochs: int = 30, max_samples: int = 200000, seed: int = 1,

@@ -89,12 +89,13 @@ struct Ctx {
     DevBuf staging, list, block_counts, ctl, spill, instr_cycles, instr_probes;
     // short-side walk: probe totals of both walks and the walk rows, a
     // per-graph index made the first time a count needs it
-    DevBuf side_cnt, side_bsum, side_beg, side_adj, side_stats;
+    DevBuf side_cnt, side_bsum, side_beg, side_adj, side_row, side_deg, side_stats;
     bool side_stats_ready = false, side_built = false;
     uint64_t side_w_ref = 0, side_w_short = 0;
     int64_t side_entries = 0;  // entries of the walk rows
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
+    DevBuf walk_acc;  // per-vertex sum and max of the short-side walk's list lengths
     bool mlp_ready = false;
     // the degree split as judge: which schedule counted the resident graph
     // faster when the model was trained on it (-1: not judged)
@@ -143,6 +144,8 @@ struct ResidentRun {
     double index_seconds = 0;    // one-time index work done while preparing
 };
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
+// Builds the short-side walk index of the resident graph if it has none.
+int ensure_walk_index(Ctx *ctx);
 // Releases the captured batch graph (after a new load or at teardown).
 void drop_captured(Ctx *ctx);
 // Enqueues the fills, plan and counting launches on ctx->stream; `timed`

@@ -400,6 +400,10 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
             rc = ctx->side_beg.ensure(ob * (size_t)(g.V + 1));
         if (!rc)
             rc = ctx->side_adj.ensure(sizeof(int32_t) * (size_t)g.E);
+        if (!rc)
+            rc = ctx->side_row.ensure(sizeof(int32_t) * (size_t)g.E);
+        if (!rc)
+            rc = ctx->side_deg.ensure(sizeof(int32_t) * (size_t)g.E);
         if (rc)
             return rc;
         if (g.off64)
@@ -407,13 +411,17 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
                                               g.V, (int *)ctx->side_cnt.p,
                                               (long long *)ctx->side_bsum.p,
                                               (int64_t *)ctx->side_beg.p,
-                                              (int32_t *)ctx->side_adj.p, ctx->stream));
+                                              (int32_t *)ctx->side_adj.p,
+                                              (int32_t *)ctx->side_row.p,
+                                              (int32_t *)ctx->side_deg.p, ctx->stream));
         else
             HIPCHK(launch_side_build<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
                                               g.V, (int *)ctx->side_cnt.p,
                                               (long long *)ctx->side_bsum.p,
                                               (int32_t *)ctx->side_beg.p,
-                                              (int32_t *)ctx->side_adj.p, ctx->stream));
+                                              (int32_t *)ctx->side_adj.p,
+                                              (int32_t *)ctx->side_row.p,
+                                              (int32_t *)ctx->side_deg.p, ctx->stream));
         if (g.off64) {
             int64_t n = 0;
             HIPCHK(hipMemcpyAsync(&n, (const int64_t *)ctx->side_beg.p + g.V, sizeof(n),
@@ -432,6 +440,14 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
     R.index_seconds =
         worked ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() : 0.0;
     return TRUST_OK;
+}
+
+int ensure_walk_index(Ctx *ctx)
+{
+    if (ctx->g.V <= 0 || ctx->g.E <= 0 || ctx->side_built)
+        return TRUST_OK;
+    ResidentRun R;
+    return prepare_side(ctx, true, R);
 }
 
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)

@@ -77,7 +77,8 @@ __device__ __forceinline__ void fw_sync()
 
 template <typename OffT>
 __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ beg,
-                                                       const int32_t *__restrict__ adj, long long V,
+                                                       const int32_t *__restrict__ adj,
+                                                       const OffT *__restrict__ begD, long long V,
                                                        float *__restrict__ feat)
 {
     __shared__ int mark_s[4][64 * kFeatWin];
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
             int dv[kFeatWin];
 #pragma unroll
             for (int k = 0; k < kFeatWin; k++)
-                dv[k] = v[k] >= 0 ? (int)(beg[v[k] + 1] - beg[v[k]]) : 0;
+                dv[k] = v[k] >= 0 ? (int)(begD[v[k] + 1] - begD[v[k]]) : 0;
 #pragma unroll
             for (int k = 0; k < kFeatWin; k++) {
                 if (v[k] >= 0) {
@@ -133,11 +134,76 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
         fw_sync();
         if (lane < n) {
             const long long u = u0 + lane;
-            feat[3 * u] = log2f(1.f + (float)d) * kFeatScale;
+            const float dd = (float)(begD[u + 1] - begD[u]);  // table size d+(u)
+            feat[3 * u] = log2f(1.f + dd) * kFeatScale;
             feat[3 * u + 1] = log2f(1.f + (float)lsum[lane]) * kFeatScale;
             feat[3 * u + 2] = log2f(1.f + (float)lmax[lane]) * kFeatScale;
         }
         fw_sync();
+    }
+}
+
+// The same features over the short-side walk's rows, from the per-entry row
+// and list length the walk index stores: a flat pass, 64 consecutive entries
+// per wave step, so long rows (a hub's in-walk) spread over many waves.  Rows
+// are contiguous and in order, so each wave reduces its row segments (an
+// add scan and a max scan keyed by row) and adds one atomic per segment into
+// wsum / wmax (cleared before).
+__global__ void __launch_bounds__(256) walk_feat_kernel(const int32_t *__restrict__ rowX,
+                                                        const int32_t *__restrict__ degX,
+                                                        long long X,
+                                                        unsigned long long *__restrict__ wsum,
+                                                        unsigned int *__restrict__ wmax)
+{
+    const int lane = lane64();
+    const long long nw = (long long)gridDim.x * (blockDim.x / 64);
+    for (long long w = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w * 64 < X;
+         w += nw) {
+        const long long pos = w * 64 + lane;
+        const bool valid = pos < X;
+        const int r = valid ? rowX[pos] : -1;
+        const int dv = valid ? degX[pos] : 0;
+        const int rprev = __shfl_up(r, 1);
+        const int rnext = __shfl_down(r, 1);
+        const bool head = valid && (lane == 0 || rprev != r);
+        const bool last = valid && (lane == 63 || rnext != r);
+        int inc = dv;  // inclusive add scan (a window's sum fits an int)
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(inc, off);
+            if (lane >= off)
+                inc += y;
+        }
+        // rows do not decrease along the entries, so a max scan of
+        // (row, d+) keys is a max per row segment
+        unsigned long long key =
+            valid ? ((unsigned long long)(unsigned int)r << 32) | (unsigned int)dv : 0ull;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned long long y = __shfl_up(key, off);
+            if (lane >= off && y > key)
+                key = y;
+        }
+        const int hl = fw_scan_max(head ? lane : 0);  // this lane's segment head
+        const int before = __shfl(inc, hl > 0 ? hl - 1 : 0);
+        const int seg = hl > 0 ? inc - before : inc;
+        if (last) {
+            atomicAdd(wsum + r, (unsigned long long)seg);
+            atomicMax(wmax + r, (unsigned int)(key & 0xffffffffu));
+        }
+    }
+}
+
+template <typename OffT>
+__global__ void __launch_bounds__(256) walk_feat_final_kernel(const OffT *__restrict__ begD,
+                                                              long long V,
+                                                              const unsigned long long *__restrict__ wsum,
+                                                              const unsigned int *__restrict__ wmax,
+                                                              float *__restrict__ feat)
+{
+    for (long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x; u < V;
+         u += (long long)gridDim.x * blockDim.x) {
+        feat[3 * u] = log2f(1.f + (float)(begD[u + 1] - begD[u])) * kFeatScale;
+        feat[3 * u + 1] = log2f(1.f + (float)wsum[u]) * kFeatScale;
+        feat[3 * u + 2] = log2f(1.f + (float)wmax[u]) * kFeatScale;
     }
 }
 
@@ -231,7 +297,8 @@ __global__ void __launch_bounds__(256) keys_kernel(const OffT *__restrict__ beg,
          u += (long long)gridDim.x * blockDim.x) {
         const long long d = (long long)(beg[u + 1] - beg[u]);
         unsigned short key = kKeyNone;
-        if (d >= 2) {
+        // vertices with an empty walk row (short-side walk) have nothing to count
+        if (d >= 2 && (!begWalk || begWalk[u + 1] > begWalk[u])) {
             int lg = kMinTableLg;
             while ((1ll << lg) < 2 * d)
                 lg++;
@@ -381,23 +448,48 @@ __global__ void __launch_bounds__(256) key_scatter_kernel(const unsigned short *
 }
 
 // ------------------------------------------------------------- host side
+// Features over the walk the count takes: the rows of N+ (the reference
+// walk) or, with walk = true, the short-side walk's rows; table sizes and
+// list lengths are always d+.
 template <typename OffT>
-static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s)
+static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s, bool walk)
 {
     const long long V = ctx->g.V;
     if (V == 0)
         return TRUST_OK;
     const int blocks = (int)std::min<long long>((V + 255) / 256, (long long)ctx->num_cus * 32);
-    hipLaunchKernelGGL(features_kernel<OffT>, dim3(blocks), dim3(256), 0, s,
-                       (const OffT *)ctx->g.beg.p, (const int32_t *)ctx->g.adj.p, V, feat);
+    const OffT *begD = (const OffT *)ctx->g.beg.p;
+    if (!walk) {
+        hipLaunchKernelGGL(features_kernel<OffT>, dim3(blocks), dim3(256), 0, s, begD,
+                           (const int32_t *)ctx->g.adj.p, begD, V, feat);
+        HIPCHK(hipGetLastError());
+        return TRUST_OK;
+    }
+    int rc = ctx->walk_acc.ensure(12 * (size_t)V);
+    if (rc)
+        return rc;
+    unsigned long long *wsum = (unsigned long long *)ctx->walk_acc.p;
+    unsigned int *wmax = (unsigned int *)(wsum + V);
+    HIPCHK(launch_fill_u32((uint32_t *)ctx->walk_acc.p, 0u, 3 * V, s));
+    const long long X = ctx->side_entries;
+    if (X > 0) {
+        const int wb = (int)std::min<long long>((X + 255) / 256, (long long)ctx->num_cus * 64);
+        hipLaunchKernelGGL(walk_feat_kernel, dim3(wb), dim3(256), 0, s,
+                           (const int32_t *)ctx->side_row.p, (const int32_t *)ctx->side_deg.p, X,
+                           wsum, wmax);
+    }
+    hipLaunchKernelGGL(walk_feat_final_kernel<OffT>, dim3(blocks), dim3(256), 0, s, begD, V,
+                       (const unsigned long long *)wsum, (const unsigned int *)wmax, feat);
     HIPCHK(hipGetLastError());
     return TRUST_OK;
 }
 
-static int features_dispatch(Ctx *ctx, float *feat, hipStream_t s)
+static int features_dispatch(Ctx *ctx, float *feat, hipStream_t s, bool walk = false)
 {
-    return ctx->g.off64 ? enqueue_features<int64_t>(ctx, feat, s)
-                        : enqueue_features<int32_t>(ctx, feat, s);
+    if (walk && !ctx->side_built)
+        return fail(TRUST_ERR_ARG, "short-side walk rows not built");
+    return ctx->g.off64 ? enqueue_features<int64_t>(ctx, feat, s, walk)
+                        : enqueue_features<int32_t>(ctx, feat, s, walk);
 }
 
 template <typename OffT>
@@ -412,7 +504,7 @@ static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchP
         (rc = ctx->list.ensure(sizeof(int32_t) * std::max<long long>(V, 1))))
         return rc;
     HIPCHK(launch_fill_u32((uint32_t *)ctl, 0u, sizeof(TcCtl) / 4, s));
-    rc = enqueue_features<OffT>(ctx, (float *)ctx->feat.p, s);
+    rc = enqueue_features<OffT>(ctx, (float *)ctx->feat.p, s, begWalk != nullptr);
     if (rc)
         return rc;
     const int cb = (int)std::max<long long>(1, std::min<long long>((V + 255) / 256, 4096));
@@ -484,7 +576,8 @@ int train_cost_model(Ctx *ctx, const trust_params &p_in, int epochs, long long m
     rc = ctx->feat.ensure(sizeof(float) * 3 * V);
     if (rc)
         return rc;
-    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream);
+    // features of the walk the instrumented count took
+    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream, r.walk_used != 0);
     if (rc)
         return rc;
     std::vector<float> feat((size_t)3 * V);
@@ -583,6 +676,27 @@ extern "C" int trust_ctx_features(trust_ctx *c, float *feat)
     if (rc)
         return rc;
     rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(feat, ctx->feat.p, sizeof(float) * 3 * V, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return TRUST_OK;
+}
+
+extern "C" int trust_ctx_walk_features(trust_ctx *c, float *feat)
+{
+    if (!c || !feat)
+        return fail(TRUST_ERR_ARG, "null argument");
+    Ctx *ctx = reinterpret_cast<Ctx *>(c);
+    HIPCHK(hipSetDevice(ctx->device));
+    const long long V = ctx->g.V;
+    int rc = ctx->feat.ensure(sizeof(float) * 3 * std::max<long long>(V, 1));
+    if (rc)
+        return rc;
+    if ((rc = ensure_walk_index(ctx)))
+        return rc;
+    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream, ctx->g.V > 0 && ctx->g.E > 0);
     if (rc)
         return rc;
     HIPCHK(hipMemcpyAsync(feat, ctx->feat.p, sizeof(float) * 3 * V, hipMemcpyDeviceToHost,

@@ -965,7 +965,9 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
 //   kPass 1: kept[u] = number of kept out-entries of row u (counted in LDS,
 //            each row belongs to one wave), incnt[v] += 1 per in-entry;
 //   kPass 2: writes the rows: out-entries at begX[u] + (LDS slot), in-entries
-//            from the row's end down (incnt counts back to zero).
+//            from the row's end down (incnt counts back to zero); rowX and
+//            degX receive each entry's row and d+ (the learned plan's
+//            features stream them).
 constexpr int kSideWin = 4;
 
 template <typename OffT, int kPass>
@@ -975,6 +977,8 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                                                         int *__restrict__ kept,
                                                         const OffT *__restrict__ begX,
                                                         int32_t *__restrict__ adjX,
+                                                        int32_t *__restrict__ rowX,
+                                                        int32_t *__restrict__ degX,
                                                         unsigned long long *__restrict__ stats)
 {
     unsigned long long w_ref = 0, w_short = 0;
@@ -1032,10 +1036,16 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                         const long long u = u0 + own[k];
                         if (dv <= du) {
                             const int slot = atomicAdd(rowcnt + own[k], 1);
-                            adjX[begX[u] + (OffT)slot] = v[k];
+                            const OffT at = begX[u] + (OffT)slot;
+                            adjX[at] = v[k];
+                            rowX[at] = (int32_t)u;
+                            degX[at] = dv;
                         } else {
                             const int k2 = atomicSub(incnt + v[k], 1);
-                            adjX[begX[v[k] + 1] - (OffT)k2] = (int32_t)u;
+                            const OffT at = begX[v[k] + 1] - (OffT)k2;
+                            adjX[at] = (int32_t)u;
+                            rowX[at] = v[k];
+                            degX[at] = du;
                         }
                     }
                 }
@@ -1152,7 +1162,8 @@ __global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restri
 
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begX, int32_t *adjX, hipStream_t stream)
+                             long long *bsum, OffT *begX, int32_t *adjX, int32_t *rowX,
+                             int32_t *degX, hipStream_t stream)
 {
     if (V <= 0)
         return hipSuccess;
@@ -1164,7 +1175,7 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
         return e;
     hipLaunchKernelGGL((side_pass_kernel<OffT, 1>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, incnt, kept, (const OffT *)nullptr, (int32_t *)nullptr,
-                       (unsigned long long *)nullptr);
+                       (int32_t *)nullptr, (int32_t *)nullptr, (unsigned long long *)nullptr);
     hipLaunchKernelGGL(side_scan_reduce_kernel, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, bsum);
     hipLaunchKernelGGL(side_scan_top_kernel<OffT>, dim3(1), dim3(1024), 0, stream, bsum, nb, V,
@@ -1172,15 +1183,17 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
     hipLaunchKernelGGL(side_scan_out_kernel<OffT>, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, (const long long *)bsum, begX);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 2>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, adjX,
+                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, adjX, rowX, degX,
                        (unsigned long long *)nullptr);
     return hipGetLastError();
 }
 
 template hipError_t launch_side_build<int32_t>(const int32_t *, const int32_t *, long long, int *,
-                                               long long *, int32_t *, int32_t *, hipStream_t);
+                                               long long *, int32_t *, int32_t *, int32_t *,
+                                               int32_t *, hipStream_t);
 template hipError_t launch_side_build<int64_t>(const int64_t *, const int32_t *, long long, int *,
-                                               long long *, int64_t *, int32_t *, hipStream_t);
+                                               long long *, int64_t *, int32_t *, int32_t *,
+                                               int32_t *, hipStream_t);
 
 template <typename OffT>
 hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
@@ -1192,7 +1205,7 @@ hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
     const int pass_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 0>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, (int *)nullptr, (int *)nullptr, (const OffT *)nullptr,
-                       (int32_t *)nullptr, stats);
+                       (int32_t *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr, stats);
     return hipGetLastError();
 }
 
