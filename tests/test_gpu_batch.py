@@ -38,7 +38,8 @@ def bctx(gpu, batch):
     {}, dict(threads=64), dict(threads=1024), dict(chunk=1), dict(chunk=7),
     dict(flags=T.FLAG_FORCE_SPILL), dict(flags=T.FLAG_WAVE_ONLY), dict(hub_threshold=0),
     dict(flags=T.FLAG_LINEAR_PROBING), dict(flags=T.FLAG_OFFSETS64),
-    dict(blocks=7),
+    dict(blocks=7), dict(flags=T.FLAG_SHORT_WALK), dict(flags=T.FLAG_SHORT_WALK, threads=64),
+    dict(flags=T.FLAG_REFERENCE_WALK),
 ], ids=lambda kw: ",".join(f"{k}={v}" for k, v in kw.items()) or "default")
 def test_batch_counts_match_oracle(gpu, bctx, batch, kw):
     beg, adj, vb, want = batch
@@ -83,10 +84,13 @@ def test_hip_graph_replays_match_eager(bctx, batch):
         if i % 2 == 0:
             assert np.array_equal(bctx.count_batch(threads=512)[0], want)
             assert bctx.count().triangles == int(want.sum())
-    # a parameter change re-captures; the learned schedule replays too
-    for kw in (dict(threads=1024), dict(schedule=T.SCHED_LEARNED), {}):
+    # a parameter change re-captures; the learned schedule and the short-side
+    # walk (its walk rows built before capture) replay too
+    for kw in (dict(threads=1024), dict(schedule=T.SCHED_LEARNED),
+               dict(flags=T.FLAG_SHORT_WALK), {}):
+        rest = {k: v for k, v in kw.items() if k != "flags"}
         for _ in range(3):
-            counts, _, _ = bctx.count_batch(flags=T.FLAG_HIP_GRAPH, **kw)
+            counts, _, _ = bctx.count_batch(flags=T.FLAG_HIP_GRAPH | kw.get("flags", 0), **rest)
             assert np.array_equal(counts, want), kw
 
 

@@ -154,6 +154,12 @@ def test_rmat16_offsets64(gpu, rmat16):
         c.load(beg, adj, flags=T.FLAG_OFFSETS64)
         assert c.count().triangles == want
         assert c.count(flags=T.FLAG_FORCE_SPILL).triangles == want
+        # int64 walk rows: build, count, gather and walk features
+        r = c.count(flags=T.FLAG_SHORT_WALK)
+        assert r.triangles == want and r.probes == oracle.probes_short(beg, adj)
+        assert c.gather(flags=T.FLAG_SHORT_WALK).probes == r.probes
+        assert np.allclose(c.walk_features(), oracle.mlp_scaled_features_short(beg, adj),
+                           rtol=1e-5, atol=1e-6)
 
 
 @pytest.mark.parametrize("P", [2, 3, 8])
