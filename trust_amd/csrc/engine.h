@@ -48,6 +48,9 @@ struct CsrViews {
     const int32_t *adjW;
     const void *begP;
     const int32_t *adjP;
+    // walk entries as list start (OffT) and length (null: adjW holds ids)
+    const void *startW = nullptr;
+    const int32_t *degW = nullptr;
 };
 
 // Shape of one counting launch.

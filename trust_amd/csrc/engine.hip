@@ -222,6 +222,8 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.ctl = ctl_dev;
     a.instr_cycles = instr_cycles;
     a.instr_probes = instr_probes;
+    a.startW = (const OffT *)views.startW;
+    a.degW = views.degW;
     if (ctx->batch_ngraphs > 0) {  // read by the batch mode only
         const int64_t ng = ctx->batch_ngraphs;
         a.graph_vbase = (const long long *)ctx->batch_vbase.p;
@@ -354,10 +356,10 @@ double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
 double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t X)
 {
     // hash rows: offsets once + every N+(u); walk rows: offsets once + every
-    // entry; two offsets per walked entry; every probe
+    // entry's list start and length; every probe
     const double ob = off64 ? 8.0 : 4.0;
-    return 2.0 * ob * (double)(V + 1) + 4.0 * (double)E + 4.0 * (double)X +
-           2.0 * ob * (double)X + 4.0 * (double)W;
+    return 2.0 * ob * (double)(V + 1) + 4.0 * (double)E + (ob + 4.0) * (double)X +
+           4.0 * (double)W;
 }
 
 // The walk of an unpartitioned count: the short side when forced or when
@@ -399,7 +401,7 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
         if (!rc)
             rc = ctx->side_beg.ensure(ob * (size_t)(g.V + 1));
         if (!rc)
-            rc = ctx->side_adj.ensure(sizeof(int32_t) * (size_t)g.E);
+            rc = ctx->side_adj.ensure(ob * (size_t)g.E);  // list starts
         if (!rc)
             rc = ctx->side_row.ensure(sizeof(int32_t) * (size_t)g.E);
         if (!rc)
@@ -411,7 +413,7 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
                                               g.V, (int *)ctx->side_cnt.p,
                                               (long long *)ctx->side_bsum.p,
                                               (int64_t *)ctx->side_beg.p,
-                                              (int32_t *)ctx->side_adj.p,
+                                              (int64_t *)ctx->side_adj.p,
                                               (int32_t *)ctx->side_row.p,
                                               (int32_t *)ctx->side_deg.p, ctx->stream));
         else
@@ -512,7 +514,9 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
     int rc;
     if (R.side) {  // walk rows built by prepare_side; hash and probe rows stay N+
         v.begW = ctx->side_beg.p;
-        v.adjW = (const int32_t *)ctx->side_adj.p;
+        v.adjW = nullptr;
+        v.startW = ctx->side_adj.p;
+        v.degW = (const int32_t *)ctx->side_deg.p;
     }
     if (R.learned) {
         rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream,

@@ -87,6 +87,10 @@ struct TcArgs {
     TcCtl *ctl;
     unsigned long long *instr_cycles;  // per vertex (instrumented kernel only)
     unsigned long long *instr_probes;
+    // Walk entries stored as the start and length of the list they probe
+    // (nullable; the short-side walk index).  Then adjW is not read.
+    const OffT *startW;
+    const int32_t *degW;
 };
 
 // Planner parameters for the degree split.
@@ -128,14 +132,14 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream
 // view of the three-view kernel): row x = the v in N+(x) with
 // d+(v) <= d+(x), then the u with u -> x and d+(u) < d+(x), rows with
 // d+ < 2 contributing nothing.  cnt: 2V ints of scratch; bsum:
-// side_scan_blocks(V) + 1 long longs; begX: V + 1 offsets; adjX, rowX, degX:
-// up to E ints each (entry, its row, its list length d+).  Launches only
-// (no host sync).
+// side_scan_blocks(V) + 1 long longs; begX: V + 1 offsets; startX (OffT),
+// rowX, degX: up to E each (the start of the entry's list in adj, the row,
+// the list length d+).  Launches only (no host sync).
 constexpr int kSideScanPer = 8192;  // vertices per scan workgroup
 inline long long side_scan_blocks(long long V) { return (V + kSideScanPer - 1) / kSideScanPer; }
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begX, int32_t *adjX, int32_t *rowX,
+                             long long *bsum, OffT *begX, OffT *startX, int32_t *rowX,
                              int32_t *degX, hipStream_t stream);
 // stats[0] = probes of the reference walk, stats[1] = of the short-side walk
 // (stats: 2 unsigned long longs, cleared here).

@@ -341,6 +341,18 @@ struct Group {
 };
 
 template <typename OffT>
+__device__ __forceinline__ Group<OffT> make_group(OffT s, int deg)
+{
+    Group<OffT> g;
+    g.deg = deg;
+    g.incl = wave_scan_add(deg);
+    g.excl = g.incl - deg;
+    g.total = readlane(g.incl, 63);
+    g.base = s - (OffT)g.excl;
+    return g;
+}
+
+template <typename OffT>
 __device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT *__restrict__ begP)
 {
     OffT s = 0;
@@ -349,13 +361,26 @@ __device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT 
         s = begP[v];
         deg = (int)(begP[v + 1] - s);
     }
-    Group<OffT> g;
-    g.deg = deg;
-    g.incl = wave_scan_add(deg);
-    g.excl = g.incl - deg;
-    g.total = readlane(g.incl, 63);
-    g.base = s - (OffT)g.excl;
-    return g;
+    return make_group(s, deg);
+}
+
+// Group of 64 walk entries starting at entry `at` of the walk view.  A walk
+// index that stores each entry's list start and length (startW, degW) is
+// read in place, coalesced; otherwise the entry is a vertex id whose offsets
+// are looked up.
+template <typename OffT>
+__device__ __forceinline__ Group<OffT> walk_group(const TcArgs<OffT> &a, OffT at, bool valid)
+{
+    if (a.startW) {
+        OffT s = 0;
+        int deg = 0;
+        if (valid) {
+            s = a.startW[at];
+            deg = a.degW[at];
+        }
+        return make_group(s, deg);
+    }
+    return load_group(valid ? a.adjW[at] : 0, valid, a.begP);
 }
 
 // Probes windows [wbeg, wend) (64 positions each) of a group's
@@ -557,8 +582,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
         // window count of every 64-neighbour chunk, then an exclusive prefix
         for (int c = wave; c < nchunks; c += nwaves) {
             const int idx = c * kWave + lane;
-            const bool valid = idx < dW;
-            const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
+            const Group<OffT> g = walk_group(a, sW + (OffT)idx, idx < dW);
             if (lane == 0)
                 ctot[c] = (g.total + kWave - 1) / kWave;
             wave_probes += (unsigned long long)g.total;
@@ -606,8 +630,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
                 }
                 if (lo != cur) {
                     const int idx = lo * kWave + lane;
-                    const bool valid = idx < dW;
-                    g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
+                    g = walk_group(a, sW + (OffT)idx, idx < dW);
                     cur = lo;
                 }
                 const int cend = min(w1, ctot[lo + 1]);
@@ -626,8 +649,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
             if (c >= nchunks)
                 break;
             const int idx = c * kWave + lane;
-            const bool valid = idx < dW;
-            const Group<OffT> g = load_group(valid ? a.adjW[sW + (OffT)idx] : 0, valid, a.begP);
+            const Group<OffT> g = walk_group(a, sW + (OffT)idx, idx < dW);
             wave_probes += (unsigned long long)g.total;
             hits += probe_windows<OffT, kGlobal, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab,
                                                  lg, linear, mark);
@@ -698,8 +720,13 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     unsigned long long wave_probes = 0;
     for (int c = 0; c < dW; c += kWave) {
         const bool valid = c + lane < dW;
-        const int v = c == 0 && same_row ? head : (valid ? a.adjW[sW + (OffT)(c + lane)] : 0);
-        const Group<OffT> g = load_group(v, valid, a.begP);
+        Group<OffT> g;
+        if (a.startW) {
+            g = walk_group(a, sW + (OffT)(c + lane), valid);
+        } else {
+            const int v = c == 0 && same_row ? head : (valid ? a.adjW[sW + (OffT)(c + lane)] : 0);
+            g = load_group(v, valid, a.begP);
+        }
         wave_probes += (unsigned long long)g.total;
         hits += probe_windows<OffT, false, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab, lg,
                                            linear, mark);
@@ -965,9 +992,10 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
 //   kPass 1: kept[u] = number of kept out-entries of row u (counted in LDS,
 //            each row belongs to one wave), incnt[v] += 1 per in-entry;
 //   kPass 2: writes the rows: out-entries at begX[u] + (LDS slot), in-entries
-//            from the row's end down (incnt counts back to zero); rowX and
-//            degX receive each entry's row and d+ (the learned plan's
-//            features stream them).
+//            from the row's end down (incnt counts back to zero).  An entry x
+//            is stored as its list's start and length (startX = beg[x],
+//            degX = d+(x)), which the count reads in place of x; rowX
+//            receives the entry's row (the learned plan's features).
 constexpr int kSideWin = 4;
 
 template <typename OffT, int kPass>
@@ -976,7 +1004,7 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                                                         long long V, int *__restrict__ incnt,
                                                         int *__restrict__ kept,
                                                         const OffT *__restrict__ begX,
-                                                        int32_t *__restrict__ adjX,
+                                                        OffT *__restrict__ startX,
                                                         int32_t *__restrict__ rowX,
                                                         int32_t *__restrict__ degX,
                                                         unsigned long long *__restrict__ stats)
@@ -1022,8 +1050,12 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
 #pragma unroll
             for (int k = 0; k < kSideWin; k++) {
                 const int du = shfl_idx(d, own[k]);
+                OffT su = 0;  // beg[u], shuffled while every lane is active
+                if constexpr (kPass == 2)
+                    su = shfl_idx(s, own[k]);
                 if (v[k] >= 0 && du >= 2) {
-                    const int dv = (int)(beg[v[k] + 1] - beg[v[k]]);
+                    const OffT bv = beg[v[k]];
+                    const int dv = (int)(beg[v[k] + 1] - bv);
                     if constexpr (kPass == 0) {
                         w_ref += (unsigned long long)dv;
                         w_short += (unsigned long long)min(du, dv);
@@ -1037,13 +1069,13 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                         if (dv <= du) {
                             const int slot = atomicAdd(rowcnt + own[k], 1);
                             const OffT at = begX[u] + (OffT)slot;
-                            adjX[at] = v[k];
+                            startX[at] = bv;
                             rowX[at] = (int32_t)u;
                             degX[at] = dv;
                         } else {
                             const int k2 = atomicSub(incnt + v[k], 1);
                             const OffT at = begX[v[k] + 1] - (OffT)k2;
-                            adjX[at] = (int32_t)u;
+                            startX[at] = su;
                             rowX[at] = v[k];
                             degX[at] = du;
                         }
@@ -1162,7 +1194,7 @@ __global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restri
 
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begX, int32_t *adjX, int32_t *rowX,
+                             long long *bsum, OffT *begX, OffT *startX, int32_t *rowX,
                              int32_t *degX, hipStream_t stream)
 {
     if (V <= 0)
@@ -1174,7 +1206,7 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL((side_pass_kernel<OffT, 1>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, incnt, kept, (const OffT *)nullptr, (int32_t *)nullptr,
+                       beg, adj, V, incnt, kept, (const OffT *)nullptr, (OffT *)nullptr,
                        (int32_t *)nullptr, (int32_t *)nullptr, (unsigned long long *)nullptr);
     hipLaunchKernelGGL(side_scan_reduce_kernel, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, bsum);
@@ -1183,7 +1215,7 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
     hipLaunchKernelGGL(side_scan_out_kernel<OffT>, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, (const long long *)bsum, begX);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 2>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, adjX, rowX, degX,
+                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, startX, rowX, degX,
                        (unsigned long long *)nullptr);
     return hipGetLastError();
 }
@@ -1192,7 +1224,7 @@ template hipError_t launch_side_build<int32_t>(const int32_t *, const int32_t *,
                                                long long *, int32_t *, int32_t *, int32_t *,
                                                int32_t *, hipStream_t);
 template hipError_t launch_side_build<int64_t>(const int64_t *, const int32_t *, long long, int *,
-                                               long long *, int64_t *, int32_t *, int32_t *,
+                                               long long *, int64_t *, int64_t *, int32_t *,
                                                int32_t *, hipStream_t);
 
 template <typename OffT>
@@ -1205,7 +1237,7 @@ hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
     const int pass_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 0>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, (int *)nullptr, (int *)nullptr, (const OffT *)nullptr,
-                       (int32_t *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr, stats);
+                       (OffT *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr, stats);
     return hipGetLastError();
 }
 
