@@ -768,15 +768,21 @@ __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int
     }
 }
 
-// Eight waves per SIMD: the compiler fits the kernel in 64 VGPRs (a few
-// spills outside the probe loop).  The walk is latency-bound, and the
-// eighth wave measured 15% faster at R-MAT 24 than seven waves without
-// spills (1.6x at 1024-thread workgroups).  kBig: the large-LDS launch
+// Waves per SIMD: with int32 offsets eight, and the compiler fits the
+// kernel in 64 VGPRs (a few spills outside the probe loop).  The walk is
+// latency-bound, and the eighth wave measured 15% faster at R-MAT 24 than
+// seven waves without spills (1.6x at 1024-thread workgroups).  With int64
+// offsets (64-bit window bases and list starts) 64 VGPRs spill inside the
+// probe loop: seven waves (72 VGPRs) measured 25.4 ms on a 1000-graph batch
+// against 33.4 ms at eight, 26.3 at six.  kBig: the large-LDS launch
 // (HBM-table spill tier + big tier); otherwise the main launch (hub + wave
 // tiers).  Each is compiled with only its own phases, so the registers of
 // one do not constrain the other.
+template <typename OffT>
+constexpr int kMinWavesPerEU = sizeof(OffT) == 4 ? 8 : 7;
+
 template <typename OffT, int kMode, bool kBig>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEU<OffT>, 8)))
 tc_count_kernel(TcArgs<OffT> a)
 {
     extern __shared__ int lds[];
