@@ -287,6 +287,20 @@ int trust_ctx_load_batch(trust_ctx *ctx, const int64_t *beg, const int32_t *adj,
  */
 int trust_ctx_count_batch(trust_ctx *ctx, const trust_params *p, uint64_t *counts,
                           double *done_seconds, trust_result *r);
+/*
+ * No counterpart in the reference (which counts one graph per process).
+ * A batch with E >= 2^30 needs int64 offsets as a whole, but no edge leaves
+ * its graph, so trust_ctx_load_batch cuts it into segments of whole graphs
+ * with fewer than 2^30 edges each, whose offsets are int32 relative to the
+ * segment's first edge, and counts run the int32 kernel per segment (the
+ * short-side walk and the learned plan keep the int64 kernel).  This call
+ * re-cuts the loaded batch with at most max_edges edges per segment (capped
+ * at 2^30 - 1; 0: no segments).  Segments need int64 offsets on the device
+ * (E >= 2^30, or TRUST_FLAG_OFFSETS64 at load, which alone does not cut) and
+ * no graph above the cap; otherwise the batch stays whole.  *segments
+ * (nullable) receives the segment count, 0 when the batch is whole.
+ */
+int trust_ctx_batch_segments(trust_ctx *ctx, int64_t max_edges, int64_t *segments);
 
 /* ---- input contract: files, preprocessing, tiles ---------------------- */
 /* Read <folder>/begin.bin + adjacent.bin (Without-graph-partition/graph.hpp:18-67)

@@ -173,3 +173,97 @@ def test_graphs_complete_progressively(gpu, schedule):
         p50, p100 = np.percentile(done, 50), done.max()
         print(f"[gpu] schedule {schedule}: p50 {p50 * 1e3:.3f} ms, max {p100 * 1e3:.3f} ms", flush=True)
         assert p50 < 0.8 * p100
+
+
+def greedy_segments(beg, vb, max_edges):
+    """Segments of whole graphs with at most max_edges edges each, cut
+    greedily in graph order; 0 when some graph alone exceeds the limit."""
+    eb = np.asarray(beg)[np.asarray(vb)]
+    ng, n, g = len(vb) - 1, 0, 0
+    max_edges = min(max_edges, (1 << 30) - 1)
+    if max_edges <= 0:
+        return 0
+    while g < ng:
+        if eb[g + 1] - eb[g] > max_edges:
+            return 0
+        g0 = g
+        while g < ng and eb[g + 1] - eb[g0] <= max_edges:
+            g += 1
+        n += vb[g] > vb[g0]
+    return n
+
+
+@pytest.fixture(scope="module")
+def sctx(gpu, batch):
+    beg, adj, vb, _ = batch
+    c = T.Context(gpu)
+    c.load_batch(beg, adj, vb, flags=T.FLAG_OFFSETS64)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("limit", ["below_largest", "largest", "3x_largest", "all", "cap", "off"])
+def test_int32_segments_match_oracle(sctx, batch, limit):
+    # an int64 batch counted as int32 segments of whole graphs must give every
+    # graph its own count, whatever the cut
+    beg, adj, vb, want = batch
+    e = np.diff(beg[vb])
+    max_edges = {"below_largest": int(e.max()) - 1, "largest": int(e.max()),
+                 "3x_largest": 3 * int(e.max()), "all": len(adj), "cap": 1 << 40, "off": 0}[limit]
+    nseg = sctx.batch_segments(max_edges)
+    print(f"[gpu] int64 batch, segments of <= {max_edges} edges: {nseg}", flush=True)
+    assert nseg == greedy_segments(beg, vb, max_edges)
+    if limit in ("all", "cap"):
+        assert nseg == 1
+    if limit == "largest":
+        assert nseg > 10
+    for kw in ({}, dict(threads=1024), dict(flags=T.FLAG_FORCE_SPILL), dict(chunk=1),
+               dict(flags=T.FLAG_WAVE_ONLY), dict(flags=T.FLAG_SHORT_WALK),
+               dict(flags=T.FLAG_HIP_GRAPH)):
+        counts, done, r = sctx.count_batch(**kw)
+        assert np.array_equal(counts, want), kw
+        assert r.triangles == int(want.sum()), kw
+        assert np.all(done[want > 0] > 0) and done.max() <= r.seconds + 1e-3, kw
+    for i in range(3):  # replays with eager work between them
+        assert np.array_equal(sctx.count_batch(flags=T.FLAG_HIP_GRAPH)[0], want), i
+        r = sctx.count()
+        assert r.triangles == int(want.sum())
+    # tier sizes add up over segments: every vertex with d+ >= 2 once
+    assert r.n_spill + r.n_hub + r.n_wave == int(np.sum(np.diff(beg) >= 2))
+    assert sctx.gather().probes == r.probes  # the gather-only pass walks the same lists
+    assert sctx.count(rank=1, total_process=2).triangles + \
+        sctx.count(rank=0, total_process=2).triangles == int(want.sum())
+
+
+def test_segments_with_empty_graphs(gpu):
+    names = ["K10", "W30", "K150", "C9", "F25", "K50"]
+    parts, want = [], []
+    for n in names:
+        b, a, w = closed_form_csr(n)
+        parts += [(b, a), (np.zeros(1, np.int64), np.zeros(0, np.int32)),
+                  (np.zeros(4, np.int64), np.zeros(0, np.int32))]
+        want += [w, 0, 0]
+    beg, adj, vb = concat_csr(parts)
+    with T.Context(gpu) as c:
+        c.load_batch(beg, adj, vb, flags=T.FLAG_OFFSETS64)
+        for max_edges in (int(np.diff(beg[vb]).max()), 2 * int(np.diff(beg[vb]).max()), 1 << 30):
+            nseg = c.batch_segments(max_edges)
+            assert nseg == greedy_segments(beg, vb, max_edges) >= 1
+            for kw in ({}, dict(flags=T.FLAG_HIP_GRAPH)):
+                assert c.count_batch(**kw)[0].tolist() == want, (max_edges, kw)
+
+
+def test_batch_segments_rules(gpu):
+    beg, adj, vb = T.batch_generate(20, seed=2, nmin=50, nmax=200, dmin=4, dmax=8)
+    with T.Context(gpu) as c:
+        with pytest.raises(T.TrustError):
+            c.batch_segments(1000)  # no batch loaded
+        c.load_batch(beg, adj, vb)  # int32 offsets: nothing to cut
+        assert c.batch_segments(1000) == 0
+        c.load_batch(beg, adj, vb, flags=T.FLAG_OFFSETS64)  # the flag alone does not cut
+        assert c.batch_segments(len(adj)) == 1
+        with pytest.raises(T.TrustError):
+            c.batch_segments(-1)
+        c.load(beg, adj)  # a plain load ends the batch
+        with pytest.raises(T.TrustError):
+            c.batch_segments(1000)

@@ -134,6 +134,7 @@ SIGNATURES = {
                                        C.c_int64, C.c_uint32]),
     "trust_ctx_count_batch": (C.c_int, [C.c_void_p, C.POINTER(Params), _PU64,
                                         C.POINTER(C.c_double), C.POINTER(Result)]),
+    "trust_ctx_batch_segments": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "trust_membench_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "trust_csr_load": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "trust_csr_save": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
@@ -471,6 +472,13 @@ class Context:
                                            done.ctypes.data_as(C.POINTER(C.c_double)),
                                            C.byref(r)))
         return counts, done, r
+
+    def batch_segments(self, max_edges: int) -> int:
+        """Re-cut the resident int64 batch into int32 segments of at most
+        max_edges edges (0: count it whole); returns the segment count."""
+        n = C.c_int64(0)
+        _check(lib().trust_ctx_batch_segments(self._h, max_edges, C.byref(n)))
+        return n.value
 
     def gather(self, **params) -> Result:
         """Gather-only pass (same walk, no hashing): the load-path time."""

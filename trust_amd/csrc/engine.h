@@ -75,6 +75,28 @@ struct LaunchPlan {
 // Vertices per page of the batch's graph lookup table.
 constexpr int kGraphPageShift = 12;
 
+// A batch too large for int32 offsets is counted as segments of whole
+// graphs, each with fewer than 2^30 edges (the int32 kernel's limit), whose
+// offsets are int32 relative to the segment's first edge.
+constexpr int64_t kSegMaxEdges = (int64_t(1) << 30) - 1;
+struct BatchSeg {
+    int64_t v0 = 0, v1 = 0;   // vertices [v0, v1)
+    int64_t e0 = 0, e1 = 0;   // edges [e0, e1)
+    int64_t g0 = 0, g1 = 0;   // graphs [g0, g1)
+    int64_t beg_off = 0;      // its v1 - v0 + 1 offsets start at Ctx::seg_beg[beg_off]
+    int64_t vbase_off = 0;    // its g1 - g0 + 1 local vertex bases in Ctx::seg_vbase
+    int64_t page_off = 0;     // its page table in Ctx::seg_page
+};
+// What a counting launch of one segment reads instead of the whole batch's
+// list and graph tables (vertex ids in them are segment-local).
+struct SegView {
+    const int32_t *list;
+    const long long *graph_vbase;
+    const int *graph_page;
+    int ngraphs;
+    unsigned long long *graph_counts, *graph_done;
+};
+
 struct Ctx {
     int device = 0;
     bool ready = false;
@@ -108,6 +130,11 @@ struct Ctx {
     DevBuf batch_vbase, batch_page, batch_out;
     int64_t batch_ngraphs = 0;
     double wall_hz = 0;
+    // host copies of the batch's graph vertex and edge bases (ngraphs + 1)
+    std::vector<int64_t> batch_vbase_h, batch_ebase_h;
+    // segments of an int64 batch (empty: the int64 kernel counts it whole)
+    std::vector<BatchSeg> segs;
+    DevBuf seg_beg, seg_vbase, seg_page;
     // captured plan + count of the batch (TRUST_FLAG_HIP_GRAPH)
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uintptr_t> graph_key;
@@ -132,7 +159,8 @@ template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count);
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
+                  const SegView *seg = nullptr, bool plan_only = false);
 
 int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
@@ -144,6 +172,7 @@ struct ResidentRun {
     LaunchPlan L;
     bool learned = false;
     bool side = false;           // short-side walk
+    bool segmented = false;      // an int64 batch counted as int32 segments
     double index_seconds = 0;    // one-time index work done while preparing
 };
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
@@ -151,6 +180,13 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
 int ensure_walk_index(Ctx *ctx);
 // Releases the captured batch graph (after a new load or at teardown).
 void drop_captured(Ctx *ctx);
+// Cuts the loaded batch into segments of at most max_edges edges (0: none).
+int build_segments(Ctx *ctx, int64_t max_edges);
+// Enqueues the plans and counting launches of every segment.
+int enqueue_segments(Ctx *ctx, const trust_params &p, int mode, const ResidentRun &R, bool timed);
+// Copies the control blocks of the last count back (one, or one per
+// segment, summed) and waits for the stream.
+int read_ctl(Ctx *ctx, const ResidentRun &R, TcCtl &h);
 // Enqueues the fills, plan and counting launches on ctx->stream; `timed`
 // records ctx->ev[0..2] around plan and count (not under capture).
 int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRun &R, bool timed);

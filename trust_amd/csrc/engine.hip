@@ -196,7 +196,8 @@ int resolve_launch(Ctx *ctx, const trust_params &p, int64_t max_deg_hash, int64_
 template <typename OffT>
 static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const trust_params &p,
                       const LaunchPlan &L, const LaunchShape &s, TcCtl *ctl_dev,
-                      unsigned long long *instr_cycles, unsigned long long *instr_probes)
+                      unsigned long long *instr_cycles, unsigned long long *instr_probes,
+                      const SegView *seg)
 {
     memset(&a, 0, sizeof(a));
     a.begH = (const OffT *)views.begH;
@@ -234,13 +235,22 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
         a.graph_done = a.graph_counts + ng;
         a.graph_t0 = a.graph_counts + 2 * ng;
     }
+    if (seg) {  // one segment of a batch: its own list and graph tables, the shared t0
+        a.list = seg->list;
+        a.graph_vbase = seg->graph_vbase;
+        a.graph_page = seg->graph_page;
+        a.ngraphs = seg->ngraphs;
+        a.graph_counts = seg->graph_counts;
+        a.graph_done = seg->graph_done;
+    }
 }
 
 template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
-                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count)
+                  bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
+                  const SegView *seg, bool plan_only)
 {
     int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
     if (rc)
@@ -271,7 +281,7 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         pa.force_spill = (p.flags & TRUST_FLAG_FORCE_SPILL) ? 1 : 0;
         pa.wave_only = (p.flags & TRUST_FLAG_WAVE_ONLY) ? 1 : 0;
         pa.tier_override = nullptr;
-        pa.list = (int32_t *)ctx->list.p;
+        pa.list = seg ? (int32_t *)seg->list : (int32_t *)ctx->list.p;
         pa.block_counts = (long long *)ctx->block_counts.p;
         pa.per_block = per_block;
         pa.ctl = ctl_dev;
@@ -279,6 +289,8 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
     }
     if (ev_plan)
         HIPCHK(hipEventRecord(ev_plan, s));
+    if (plan_only)
+        return TRUST_OK;
 
     // The large-LDS launch (big + spill tiers) and the main launch (hub +
     // wave tiers) share nothing but the result counters, so the big one runs
@@ -289,7 +301,7 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
     const bool fork = L.has_big && cap == hipStreamCaptureStatusNone && ctx->aux_stream;
     TcArgs<OffT> a;
     if (L.has_big) {
-        fill_args(a, ctx, views, p, L, L.big, ctl_dev, instr_cycles, instr_probes);
+        fill_args(a, ctx, views, p, L, L.big, ctl_dev, instr_cycles, instr_probes, seg);
         a.spill_tier = L.spill_groups > 0 ? kTierSpill : -1;
         a.group_tier = kTierBig;
         a.wave_tier = -1;
@@ -303,7 +315,7 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         if (fork)
             HIPCHK(hipEventRecord(ctx->ev_join, ctx->aux_stream));
     }
-    fill_args(a, ctx, views, p, L, L.main, ctl_dev, instr_cycles, instr_probes);
+    fill_args(a, ctx, views, p, L, L.main, ctl_dev, instr_cycles, instr_probes, seg);
     a.spill_tier = -1;
     a.group_tier = kTierHub;
     a.wave_tier = kTierWave;
@@ -318,11 +330,11 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
 template int enqueue_count<int32_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t);
+                                    hipEvent_t, hipEvent_t, const SegView *, bool);
 template int enqueue_count<int64_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t);
+                                    hipEvent_t, hipEvent_t, const SegView *, bool);
 
 int check_params(const trust_params &p)
 {
@@ -465,12 +477,6 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
                     (long long)g.max_deg);
     if (mode == kModeBatch && (ctx->batch_ngraphs < 1 || !ctx->batch_vbase.p))
         return fail(TRUST_ERR_ARG, "context holds no batch");
-    rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64, R.L, mode);
-    if (rc)
-        return rc;
-    rc = ctx->ctl.ensure(sizeof(TcCtl));
-    if (rc)
-        return rc;
     R.side = false;
     R.index_seconds = 0;
     if (!(p.flags & TRUST_FLAG_REFERENCE_WALK) && g.V > 0 && g.E > 0) {
@@ -490,6 +496,17 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
     // found the degree split faster on this graph), else the degree split
     R.learned = ctx->mlp_ready && (p.schedule == TRUST_SCHED_LEARNED ||
                                    (p.schedule == TRUST_SCHED_AUTO && ctx->judge != TRUST_SCHED_DEGREE));
+    // An int64 batch with segments runs the int32 kernel per segment; the
+    // short-side walk and the learned plan are built over the whole batch
+    // and keep the int64 kernel, and so does the instrumented mode, whose
+    // outputs are indexed by vertex.
+    R.segmented = !ctx->segs.empty() && mode != kModeInstr && !R.side && !R.learned;
+    rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64 && !R.segmented, R.L, mode);
+    if (rc)
+        return rc;
+    rc = ctx->ctl.ensure(sizeof(TcCtl) * (R.segmented ? ctx->segs.size() : 1));
+    if (rc)
+        return rc;
     return TRUST_OK;
 }
 
@@ -507,6 +524,8 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
         HIPCHK(launch_fill_u32(out, 0u, 4 * ctx->batch_ngraphs, ctx->stream));
         HIPCHK(launch_fill_u32(out + 4 * ctx->batch_ngraphs, 0xffffffffu, 2, ctx->stream));
     }
+    if (R.segmented)
+        return enqueue_segments(ctx, p, mode, R, timed);
     CsrViews v{g.beg.p, (const int32_t *)g.adj.p, g.beg.p, (const int32_t *)g.adj.p, g.beg.p,
                (const int32_t *)g.adj.p};
     if (timed)
@@ -556,7 +575,7 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
     r->h2d_seconds = ctx->h2d_seconds;
     r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
     r->alg_bytes = R.side ? alg_bytes_short(g.V, g.E, h.probes, g.off64, ctx->side_entries)
-                          : alg_bytes(g.V, g.E, h.probes, g.off64);
+                          : alg_bytes(g.V, g.E, h.probes, g.off64 && !R.segmented);
     r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
     r->n_hub = h.list_begin[kTierHub + 1] - h.list_begin[kTierBig];  // big + hub
     r->n_wave = h.list_begin[kTierWave + 1] - h.list_begin[kTierWave];
@@ -583,7 +602,6 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
         return rc;
     const DevCsr &g = ctx->g;
     TcCtl h;
-    HIPCHK(hipMemcpyAsync(&h, ctx->ctl.p, sizeof(TcCtl), hipMemcpyDeviceToHost, ctx->stream));
     if (mode == kModeInstr) {
         if (cycles_out)
             HIPCHK(hipMemcpyAsync(cycles_out, ctx->instr_cycles.p,
@@ -594,7 +612,8 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
                                   sizeof(unsigned long long) * g.V, hipMemcpyDeviceToHost,
                                   ctx->stream));
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((rc = read_ctl(ctx, R, h)))
+        return rc;
     float t_plan = 0, t_count = 0;
     HIPCHK(hipEventElapsedTime(&t_plan, ctx->ev[0], ctx->ev[1]));
     HIPCHK(hipEventElapsedTime(&t_count, ctx->ev[1], ctx->ev[2]));
@@ -661,6 +680,7 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
     // state derived from the previous graph
     drop_captured(ctx);
     ctx->batch_ngraphs = 0;
+    ctx->segs.clear();
     ctx->judge = -1;  // the judgement belongs to the graph; the model stays
     ctx->side_stats_ready = ctx->side_built = false;  // the walk index belongs to the graph
     auto t0 = std::chrono::steady_clock::now();

@@ -151,8 +151,14 @@ hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
 // more; measured break-even near 0.45-0.5 on R-MAT).
 constexpr double kShortWalkShare = 0.45;
 
+// Per-wave LDS ints after the tables: a 64-int mark array, then the wave's
+// graph state of the batch mode.
+constexpr int kMarkStride = kWave + 4;
 // LDS ints besides the tables: per-wave marks, misc words, chunk totals.
-inline int lds_extra_words(int threads) { return (threads / kWave) * kWave + 8 + kMaxChunks + 8; }
+inline int lds_extra_words(int threads)
+{
+    return (threads / kWave) * kMarkStride + 8 + kMaxChunks + 8;
+}
 
 // log2 of the smallest table for a hash source of degree d: >= 2d slots, >= 64.
 inline int host_table_lg(long long d)

@@ -99,18 +99,15 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
     return x;
 }
 
-// Per-wave attribution of triangles to the graphs of a batch.  The wave
-// keeps the vertex range of its current graph and the wave total of its
-// running count when it entered that graph, all wave-uniform (scalar
-// registers).  Entering a vertex of another graph adds the difference to the
-// previous graph, with a completion time stamp.  Called with the whole wave
-// active, u wave-uniform, before the vertex's hits are added to acc.
-struct GraphAcc {
-    int g = -1;
-    int lo = 0, hi = 0;
-    unsigned long long base = 0;  // wave total of acc when g was entered
-};
-
+// Per-wave attribution of triangles to the graphs of a batch.  The state
+// sits in LDS, in the kMarkStride - kWave words after the wave's mark array
+// (gs[0], gs[1]) = the vertex range of the graph being counted, gs[2] = that
+// graph (-1: none), not in scalar registers, where it would stay live across
+// the whole probe walk and push other values out.  The wave's running count
+// acc holds the current graph's triangles only: entering a vertex of another
+// graph adds acc to the previous graph, with a completion time stamp, and
+// clears it.  Called with the whole wave active, u wave-uniform, before the
+// vertex's hits are added to acc.
 __device__ __forceinline__ unsigned long long uniform(unsigned long long x)
 {
     const int lo = __builtin_amdgcn_readfirstlane((int)(x & 0xffffffffu));
@@ -118,26 +115,34 @@ __device__ __forceinline__ unsigned long long uniform(unsigned long long x)
     return ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
 }
 
-template <typename Args>
-__device__ __forceinline__ void graph_flush(const Args &a, GraphAcc &ga, unsigned long long acc)
+__device__ __forceinline__ void graph_init(int *gs)
 {
-    const unsigned long long total = uniform(wave_sum(acc));
-    if (ga.g >= 0 && __lane_id() == 0) {
-        if (total != ga.base)
-            atomicAdd(a.graph_counts + ga.g, total - ga.base);
-        atomicMax(a.graph_done + ga.g, (unsigned long long)wall_clock64());
-    }
-    ga.base = total;
-    ga.g = -1;
+    gs[0] = 0;
+    gs[1] = 0;  // an empty range: the first vertex enters its graph
+    gs[2] = -1;
 }
 
 template <typename Args>
-__device__ __forceinline__ void graph_enter(const Args &a, GraphAcc &ga, int u,
-                                            unsigned long long acc)
+__device__ __forceinline__ void graph_flush(const Args &a, int *gs, unsigned long long &acc)
 {
-    if (ga.g >= 0 && u >= ga.lo && u < ga.hi)
+    const unsigned long long total = uniform(wave_sum(acc));
+    const int g = __builtin_amdgcn_readfirstlane(gs[2]);
+    if (g >= 0 && __lane_id() == 0) {
+        if (total)
+            atomicAdd(a.graph_counts + g, total);
+        atomicMax(a.graph_done + g, (unsigned long long)wall_clock64());
+    }
+    acc = 0;
+}
+
+template <typename Args>
+__device__ __forceinline__ void graph_enter(const Args &a, int *gs, int u,
+                                            unsigned long long &acc)
+{
+    const int2 range = *(const int2 *)gs;
+    if (u >= __builtin_amdgcn_readfirstlane(range.x) && u < __builtin_amdgcn_readfirstlane(range.y))
         return;
-    graph_flush(a, ga, acc);
+    graph_flush(a, gs, acc);
     // the page of u bounds the search: vbase[lo] <= u < vbase[hi]
     const int pg = u >> a.graph_page_shift;
     int lo = a.graph_page[pg], hi = a.graph_page[pg + 1] + 1;
@@ -148,9 +153,10 @@ __device__ __forceinline__ void graph_enter(const Args &a, GraphAcc &ga, int u,
         else
             hi = mid;
     }
-    ga.g = __builtin_amdgcn_readfirstlane(lo);
-    ga.lo = __builtin_amdgcn_readfirstlane((int)a.graph_vbase[ga.g]);
-    ga.hi = __builtin_amdgcn_readfirstlane((int)a.graph_vbase[ga.g + 1]);
+    const int g = __builtin_amdgcn_readfirstlane(lo);
+    gs[0] = (int)a.graph_vbase[g];  // every lane stores the same values
+    gs[1] = (int)a.graph_vbase[g + 1];
+    gs[2] = g;
 }
 
 // ---------------------------------------------------------- hash tables
@@ -520,13 +526,12 @@ __device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g
 template <typename OffT, bool kGlobal, int kMode>
 __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *tab, int cap_lg,
                                              int *mark, int *misc, int *ctot,
-                                             unsigned long long &acc, unsigned long long &probes,
-                                             GraphAcc &ga)
+                                             unsigned long long &acc, unsigned long long &probes)
 {
     constexpr bool kInstr = kMode == kModeInstr;
     constexpr bool kTable = kMode != kModeGather;
     if constexpr (kMode == kModeBatch)
-        graph_enter(a, ga, u, acc);
+        graph_enter(a, mark + kWave, u, acc);
     unsigned long long t0 = 0;
     if (kInstr && threadIdx.x == 0)
         t0 = clock64();
@@ -669,13 +674,12 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
 // One wavefront counts vertex u with its own LDS table slice.
 template <typename OffT, int kMode>
 __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *tab, int *mark,
-                                            unsigned long long &acc, unsigned long long &probes,
-                                            GraphAcc &ga)
+                                            unsigned long long &acc, unsigned long long &probes)
 {
     constexpr bool kInstr = kMode == kModeInstr;
     constexpr bool kTable = kMode != kModeGather;
     if constexpr (kMode == kModeBatch)
-        graph_enter(a, ga, u, acc);
+        graph_enter(a, mark + kWave, u, acc);
     unsigned long long t0 = 0;
     if (kInstr)
         t0 = clock64();
@@ -746,8 +750,7 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
 template <typename OffT, int kMode, bool kGlobal>
 __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int *tab, int cap_lg,
                                             int *mark, int *misc, int *ctot,
-                                            unsigned long long &acc, unsigned long long &probes,
-                                            GraphAcc &ga)
+                                            unsigned long long &acc, unsigned long long &probes)
 {
     const long long lb = a.ctl->list_begin[tier], le = a.ctl->list_begin[tier + 1];
     if (le <= lb)
@@ -764,7 +767,7 @@ __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int
         const long long last = min(first + (long long)a.chunk_group, le);
         for (long long i = first; i < last; i++)
             group_vertex<OffT, kGlobal, kMode>(a, a.list[i], tab, cap_lg, mark, misc, ctot, acc,
-                                                probes, ga);
+                                                probes);
     }
 }
 
@@ -789,26 +792,27 @@ tc_count_kernel(TcArgs<OffT> a)
     const int nwaves = blockDim.x / kWave;
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
-    int *mark = lds + a.table_words + wave * kWave;
-    int *misc = lds + a.table_words + nwaves * kWave;  // 8 words
-    int *ctot = misc + 8;                               // kMaxChunks + 1 words
+    int *mark = lds + a.table_words + wave * kMarkStride;  // + the graph state (batch mode)
+    int *misc = lds + a.table_words + nwaves * kMarkStride;  // 8 words
+    int *ctot = misc + 8;                                     // kMaxChunks + 1 words
     mark[lane] = 0;
     unsigned long long acc = 0, probes = 0;
-    GraphAcc ga;
-    if (kMode == kModeBatch && threadIdx.x == 0)
-        atomicMin(a.graph_t0, (unsigned long long)wall_clock64());
+    if constexpr (kMode == kModeBatch) {
+        graph_init(mark + kWave);
+        if (threadIdx.x == 0)
+            atomicMin(a.graph_t0, (unsigned long long)wall_clock64());
+    }
 
     if (kBig && a.spill_tier >= 0 && (int)blockIdx.x < a.spill_groups) {
         int *tab = a.spill_mem + (long long)blockIdx.x * a.spill_slots;
         int cap_lg = 0;
         while ((2ll << cap_lg) <= a.spill_slots)
             cap_lg++;
-        group_phase<OffT, kMode, true>(a, a.spill_tier, tab, cap_lg, mark, misc, ctot, acc, probes,
-                                       ga);
+        group_phase<OffT, kMode, true>(a, a.spill_tier, tab, cap_lg, mark, misc, ctot, acc, probes);
     }
     if (a.group_tier >= 0)
         group_phase<OffT, kMode, false>(a, a.group_tier, lds, a.group_lg, mark, misc, ctot, acc,
-                                        probes, ga);
+                                        probes);
     __syncthreads();  // the wave tables reuse the group table's LDS
     if (!kBig && a.wave_tier >= 0) {
         const long long lb = a.ctl->list_begin[a.wave_tier];
@@ -835,11 +839,11 @@ tc_count_kernel(TcArgs<OffT> a)
             }
             const long long last = min(first + (long long)a.chunk_wave, le);
             for (long long i = first; i < last; i++)
-                wave_vertex<OffT, kMode>(a, a.list[i], wtab, mark, acc, probes, ga);
+                wave_vertex<OffT, kMode>(a, a.list[i], wtab, mark, acc, probes);
         }
     }
     if constexpr (kMode == kModeBatch)
-        graph_flush(a, ga, acc);
+        graph_flush(a, mark + kWave, acc);  // acc is now 0: the batch total is the graphs' sum
     acc = wave_sum(acc);
     probes = wave_sum(probes) / kWave;  // every lane carried the wave's probe total
     if (lane == 0) {
