@@ -52,6 +52,13 @@ extern "C" {
  * take the reference walk.  Both flags together are rejected. */
 #define TRUST_FLAG_REFERENCE_WALK 128u
 #define TRUST_FLAG_SHORT_WALK 256u
+/* The reference walk of a resident single graph with int32 offsets reads
+ * each walked v's list start and length from a per-graph start index
+ * (start[e] = beg[adj[e]], len[e] = d+(adj[e]), read in order with the walk)
+ * instead of looking up beg[v] and beg[v + 1] at random; built once, the
+ * first time a count uses it, and reported in trust_result.index_seconds.
+ * Batches and int64 offsets keep the lookups.  This flag keeps them too. */
+#define TRUST_FLAG_OFFSET_LOOKUP 512u
 /* Any other bit in trust_params.flags is rejected with TRUST_ERR_ARG. */
 
 /*

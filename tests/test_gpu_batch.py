@@ -267,3 +267,12 @@ def test_batch_segments_rules(gpu):
         c.load(beg, adj)  # a plain load ends the batch
         with pytest.raises(T.TrustError):
             c.batch_segments(1000)
+
+
+def test_segments_offset_lookup_flag(sctx, batch):
+    # batches always look the offsets up; the flag is accepted and changes nothing
+    beg, adj, vb, want = batch
+    sctx.batch_segments(int(np.diff(beg[vb]).max()))
+    for kw in (dict(flags=T.FLAG_OFFSET_LOOKUP), dict(flags=T.FLAG_HIP_GRAPH | T.FLAG_OFFSET_LOOKUP)):
+        counts, _, r = sctx.count_batch(**kw)
+        assert np.array_equal(counts, want), kw

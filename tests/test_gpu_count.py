@@ -143,7 +143,7 @@ def test_default_walk_choice_rmat21(gpu):
 
 
 def test_unknown_flags_rejected(ctx16):
-    for bad in (32, 512, 1 << 31):
+    for bad in (32, 1024, 1 << 31):
         with pytest.raises(T.TrustError):
             ctx16.count(flags=bad)
 
@@ -282,3 +282,45 @@ def test_drop_in_entry_point_and_cli(gpu, tmp_path, rmat16):
     assert fields[0] == folder
     assert int(fields[1]) == len(beg) - 1 and int(fields[2]) == len(adj)
     assert int(fields[3]) == want
+
+
+@pytest.mark.parametrize("kw", [{}, dict(threads=1024), dict(flags=T.FLAG_FORCE_SPILL),
+                                dict(flags=T.FLAG_WAVE_ONLY), dict(chunk=1),
+                                dict(flags=T.FLAG_LINEAR_PROBING)],
+                         ids=lambda kw: ",".join(f"{k}={v}" for k, v in kw.items()) or "default")
+def test_start_index_matches_offset_lookup(gpu, kw):
+    # the reference walk reading list starts from the start index must count
+    # and probe exactly what the walk that looks up the offsets does
+    beg, adj = T.rmat_csr(14, 16, seed=9)
+    want = oracle.count_hash(beg, adj)
+    w = oracle.probes(beg, adj)
+    flags = kw.get("flags", 0) | T.FLAG_REFERENCE_WALK
+    rest = {k: v for k, v in kw.items() if k != "flags"}
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        first = c.count(flags=flags, **rest)
+        assert first.index_seconds > 0  # built here, once
+        again = c.count(flags=flags, **rest)
+        assert again.index_seconds == 0
+        look = c.count(flags=flags | T.FLAG_OFFSET_LOOKUP, **rest)
+        for r in (first, again, look):
+            assert r.triangles == want and r.probes == w and r.walk_used == 0
+        g1 = c.gather(flags=flags, **rest)
+        g2 = c.gather(flags=flags | T.FLAG_OFFSET_LOOKUP, **rest)
+        assert g1.probes == g2.probes == w
+        assert g1.triangles == g2.triangles  # same lists read: same checksum
+
+
+def test_start_index_default_walk_graphs(gpu):
+    # graphs whose automatic walk is the reference one (balanced degrees)
+    # take the start index by default
+    for n, m, seed in ((3000, 40000, 1), (20000, 160000, 2)):
+        rng = np.random.default_rng(seed)
+        beg, adj = T.preprocess_edges(rng.integers(0, n, m).astype(np.int32),
+                                      rng.integers(0, n, m).astype(np.int32))
+        with T.Context(gpu) as c:
+            c.load(beg, adj)
+            r = c.count()
+            assert r.walk_used == 0
+            assert r.triangles == oracle.count_hash(beg, adj)
+            assert r.triangles == c.count(flags=T.FLAG_OFFSET_LOOKUP).triangles

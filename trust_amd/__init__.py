@@ -32,6 +32,7 @@ FLAG_FORCE_SPILL, FLAG_OFFSETS64, FLAG_NO_VALIDATE, FLAG_WAVE_ONLY, FLAG_LINEAR_
 FLAG_HIP_GRAPH = 64
 FLAG_REFERENCE_WALK = 128
 FLAG_SHORT_WALK = 256
+FLAG_OFFSET_LOOKUP = 512
 ORDER_OUTDEGREE, ORDER_PLAIN, ORDER_INDEGREE = 0, 1, 2
 ORDERS = {"outdegree": ORDER_OUTDEGREE, "plain": ORDER_PLAIN, "indegree": ORDER_INDEGREE}
 PRE_SELF_LOOP_ONCE = 1

@@ -121,6 +121,10 @@ struct Ctx {
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     DevBuf walk_acc;  // per-vertex sum and max of the short-side walk's list lengths
+    // start index of the reference walk (single graphs, int32 offsets): per
+    // entry of adj, its list start and length
+    DevBuf ref_start, ref_deg;
+    bool ref_built = false;
     bool mlp_ready = false;
     // the degree split as judge: which schedule counted the resident graph
     // faster when the model was trained on it (-1: not judged)
@@ -154,6 +158,9 @@ int check_params(const trust_params &p);
 double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64);
 // The same for the short-side walk, whose walk rows hold X entries.
 double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t X);
+// The reference walk with the start index: every entry's start and length
+// read in order instead of two offsets per walked vertex.
+double alg_bytes_starts(int64_t V, int64_t E, uint64_t W, bool off64);
 
 template <typename OffT>
 int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
@@ -173,9 +180,12 @@ struct ResidentRun {
     bool learned = false;
     bool side = false;           // short-side walk
     bool segmented = false;      // an int64 batch counted as int32 segments
+    bool starts = false;         // the reference walk reads the start index
     double index_seconds = 0;    // one-time index work done while preparing
 };
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
+// Builds the reference walk's start index unless it exists.
+int ensure_start_index(Ctx *ctx, const ResidentRun &R);
 // Builds the short-side walk index of the resident graph if it has none.
 int ensure_walk_index(Ctx *ctx);
 // Releases the captured batch graph (after a new load or at teardown).

@@ -350,7 +350,8 @@ int check_params(const trust_params &p)
         return fail(TRUST_ERR_ARG, "unknown schedule %d", p.schedule);
     const uint32_t known = TRUST_FLAG_FORCE_SPILL | TRUST_FLAG_OFFSETS64 | TRUST_FLAG_NO_VALIDATE |
                            TRUST_FLAG_WAVE_ONLY | TRUST_FLAG_LINEAR_PROBING | TRUST_FLAG_HIP_GRAPH |
-                           TRUST_FLAG_REFERENCE_WALK | TRUST_FLAG_SHORT_WALK;
+                           TRUST_FLAG_REFERENCE_WALK | TRUST_FLAG_SHORT_WALK |
+                           TRUST_FLAG_OFFSET_LOOKUP;
     if (p.flags & ~known)
         return fail(TRUST_ERR_ARG, "unknown flag bits 0x%x", p.flags & ~known);
     if ((p.flags & TRUST_FLAG_REFERENCE_WALK) && (p.flags & TRUST_FLAG_SHORT_WALK))
@@ -363,6 +364,12 @@ double alg_bytes(int64_t V, int64_t E, uint64_t W, bool off64)
     // offsets once + every N+(u) once + two offsets per walked v + every probe
     const double ob = off64 ? 8.0 : 4.0;
     return ob * (double)(V + 1) + 4.0 * (double)E + 2.0 * ob * (double)E + 4.0 * (double)W;
+}
+
+double alg_bytes_starts(int64_t V, int64_t E, uint64_t W, bool off64)
+{
+    const double ob = off64 ? 8.0 : 4.0;
+    return ob * (double)(V + 1) + 4.0 * (double)E + (ob + 4.0) * (double)E + 4.0 * (double)W;
 }
 
 double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t X)
@@ -507,6 +514,42 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
     rc = ctx->ctl.ensure(sizeof(TcCtl) * (R.segmented ? ctx->segs.size() : 1));
     if (rc)
         return rc;
+    // Single graphs with int32 offsets.  Batches keep the lookups: measured
+    // on the 10k ER/BA batch (7 int32 segments) the start index made the
+    // gather-only pass faster (162 -> 152 ms) but the count slower (207 ->
+    // 210 ms): its many small vertices are bound by instruction issue, and
+    // the lookup walk reuses the table's first 64 ids as walk ids for free.
+    R.starts = !R.side && !(p.flags & TRUST_FLAG_OFFSET_LOOKUP) && g.V > 0 && g.E > 0 &&
+               !g.off64 && ctx->batch_ngraphs == 0;
+    if (R.starts) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const bool built0 = ctx->ref_built;
+        if ((rc = ensure_start_index(ctx, R)))
+            return rc;
+        if (!built0)
+            R.index_seconds +=
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return TRUST_OK;
+}
+
+int ensure_start_index(Ctx *ctx, const ResidentRun &R)
+{
+    const DevCsr &g = ctx->g;
+    if (ctx->ref_built)
+        return TRUST_OK;
+    if (g.off64 || R.segmented)
+        return fail(TRUST_ERR_ARG, "start index of an int64 layout");
+    int rc = ctx->ref_start.ensure(sizeof(int32_t) * (size_t)g.E);
+    if (!rc)
+        rc = ctx->ref_deg.ensure(sizeof(int32_t) * (size_t)g.E);
+    if (rc)
+        return rc;
+    HIPCHK(launch_start_index<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p, g.E,
+                                       (int32_t *)ctx->ref_start.p, (int32_t *)ctx->ref_deg.p,
+                                       ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->ref_built = true;
     return TRUST_OK;
 }
 
@@ -536,6 +579,9 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
         v.adjW = nullptr;
         v.startW = ctx->side_adj.p;
         v.degW = (const int32_t *)ctx->side_deg.p;
+    } else if (R.starts) {  // walk rows = N+, each entry's list read from the start index
+        v.startW = ctx->ref_start.p;
+        v.degW = (const int32_t *)ctx->ref_deg.p;
     }
     if (R.learned) {
         rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream,
@@ -575,6 +621,7 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
     r->h2d_seconds = ctx->h2d_seconds;
     r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
     r->alg_bytes = R.side ? alg_bytes_short(g.V, g.E, h.probes, g.off64, ctx->side_entries)
+                 : R.starts ? alg_bytes_starts(g.V, g.E, h.probes, g.off64 && !R.segmented)
                           : alg_bytes(g.V, g.E, h.probes, g.off64 && !R.segmented);
     r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
     r->n_hub = h.list_begin[kTierHub + 1] - h.list_begin[kTierBig];  // big + hub
@@ -681,6 +728,7 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
     drop_captured(ctx);
     ctx->batch_ngraphs = 0;
     ctx->segs.clear();
+    ctx->ref_built = false;  // the start index belongs to the graph
     ctx->judge = -1;  // the judgement belongs to the graph; the model stays
     ctx->side_stats_ready = ctx->side_built = false;  // the walk index belongs to the graph
     auto t0 = std::chrono::steady_clock::now();

@@ -141,6 +141,11 @@ template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
                              long long *bsum, OffT *begX, OffT *startX, int32_t *rowX,
                              int32_t *degX, hipStream_t stream);
+// Start index of the reference walk: for e < E, startR[e] = begP[adj[e]]
+// and degR[e] = its list length.  Launches only.
+template <typename OffT>
+hipError_t launch_start_index(const OffT *begP, const int32_t *adj, long long E, OffT *startR,
+                              int32_t *degR, hipStream_t stream);
 // stats[0] = probes of the reference walk, stats[1] = of the short-side walk
 // (stats: 2 unsigned long longs, cleared here).
 template <typename OffT>

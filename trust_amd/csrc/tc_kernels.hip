@@ -1251,6 +1251,38 @@ hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
     return hipGetLastError();
 }
 
+template <typename OffT>
+__global__ void __launch_bounds__(256) start_index_kernel(const OffT *__restrict__ begP,
+                                                          const int32_t *__restrict__ adj,
+                                                          long long E, OffT *__restrict__ startR,
+                                                          int32_t *__restrict__ degR)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        const int v = adj[e];
+        const OffT s = begP[v];
+        startR[e] = s;
+        degR[e] = (int32_t)(begP[v + 1] - s);
+    }
+}
+
+template <typename OffT>
+hipError_t launch_start_index(const OffT *begP, const int32_t *adj, long long E, OffT *startR,
+                              int32_t *degR, hipStream_t stream)
+{
+    if (E <= 0)
+        return hipSuccess;
+    const int blocks = (int)min((E + 255) / 256, 32768ll);
+    hipLaunchKernelGGL((start_index_kernel<OffT>), dim3(blocks), dim3(256), 0, stream, begP, adj, E,
+                       startR, degR);
+    return hipGetLastError();
+}
+
+template hipError_t launch_start_index<int32_t>(const int32_t *, const int32_t *, long long,
+                                                int32_t *, int32_t *, hipStream_t);
+template hipError_t launch_start_index<int64_t>(const int64_t *, const int32_t *, long long,
+                                                int64_t *, int32_t *, hipStream_t);
+
 template hipError_t launch_side_stats<int32_t>(const int32_t *, const int32_t *, long long,
                                                unsigned long long *, hipStream_t);
 template hipError_t launch_side_stats<int64_t>(const int64_t *, const int32_t *, long long,
