@@ -1004,8 +1004,10 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
 //   kPass 2: writes the rows: out-entries at begX[u] + (LDS slot), in-entries
 //            from the row's end down (incnt counts back to zero).  An entry x
 //            is stored as its list's start and length (startX = beg[x],
-//            degX = d+(x)), which the count reads in place of x; rowX
-//            receives the entry's row (the learned plan's features).
+//            degX = d+(x)), which the count reads in place of x.
+// The entries' rows (rowX, the learned plan's features) follow from begX
+// alone and are written afterwards, in order, by side_rows_kernel: in the
+// scatter pass they were a third of the in-entries' random writes.
 constexpr int kSideWin = 4;
 
 template <typename OffT, int kPass>
@@ -1015,7 +1017,6 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                                                         int *__restrict__ kept,
                                                         const OffT *__restrict__ begX,
                                                         OffT *__restrict__ startX,
-                                                        int32_t *__restrict__ rowX,
                                                         int32_t *__restrict__ degX,
                                                         unsigned long long *__restrict__ stats)
 {
@@ -1080,13 +1081,11 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                             const int slot = atomicAdd(rowcnt + own[k], 1);
                             const OffT at = begX[u] + (OffT)slot;
                             startX[at] = bv;
-                            rowX[at] = (int32_t)u;
                             degX[at] = dv;
                         } else {
                             const int k2 = atomicSub(incnt + v[k], 1);
                             const OffT at = begX[v[k] + 1] - (OffT)k2;
                             startX[at] = su;
-                            rowX[at] = v[k];
                             degX[at] = du;
                         }
                     }
@@ -1202,6 +1201,44 @@ __global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restri
     }
 }
 
+// rowX[at] = the row whose entries [begX[u], begX[u+1]) hold position at.
+// A wave takes 64 consecutive rows, whose entries are contiguous, and
+// writes them in 64-wide windows: each non-empty row marks its first
+// position in LDS, and a max-scan carries the owner across the window.
+template <typename OffT>
+__global__ void __launch_bounds__(256) side_rows_kernel(const OffT *__restrict__ begX, long long V,
+                                                        int32_t *__restrict__ rowX)
+{
+    __shared__ int mark_s[4][kWave];
+    const int wave = threadIdx.x / kWave, lane = lane_id();
+    int *mark = mark_s[wave];
+    mark[lane] = 0;
+    const long long step = (long long)gridDim.x * (blockDim.x / kWave) * kWave;
+    for (long long u0 = ((long long)blockIdx.x * (blockDim.x / kWave) + wave) * kWave; u0 < V;
+         u0 += step) {
+        const int n = (int)min((long long)kWave, V - u0);
+        // lane i < n: row u0 + i is [s, e); lane n (if any) holds the range's end in s
+        const OffT s = begX[u0 + min(lane, n)];
+        const OffT e = lane < n ? begX[u0 + lane + 1] : s;
+        const long long p0 = (long long)readlane(s, 0);
+        const long long p1 = n == kWave ? (long long)readlane(e, 63) : (long long)readlane(s, n);
+        wave_sync();
+        int carry = 0;
+        for (long long w = p0; w < p1; w += kWave) {
+            if (lane < n && e > s && (long long)s >= w && (long long)s < w + kWave)
+                mark[(int)((long long)s - w)] = lane;
+            wave_sync();
+            const int own = max(wave_scan_max(mark[lane]), carry);
+            carry = readlane(own, 63);
+            wave_sync();
+            mark[lane] = 0;
+            if (w + lane < p1)
+                rowX[w + lane] = (int32_t)(u0 + own);
+            wave_sync();
+        }
+    }
+}
+
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
                              long long *bsum, OffT *begX, OffT *startX, int32_t *rowX,
@@ -1217,7 +1254,7 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
         return e;
     hipLaunchKernelGGL((side_pass_kernel<OffT, 1>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, incnt, kept, (const OffT *)nullptr, (OffT *)nullptr,
-                       (int32_t *)nullptr, (int32_t *)nullptr, (unsigned long long *)nullptr);
+                       (int32_t *)nullptr, (unsigned long long *)nullptr);
     hipLaunchKernelGGL(side_scan_reduce_kernel, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, bsum);
     hipLaunchKernelGGL(side_scan_top_kernel<OffT>, dim3(1), dim3(1024), 0, stream, bsum, nb, V,
@@ -1225,8 +1262,13 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
     hipLaunchKernelGGL(side_scan_out_kernel<OffT>, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, (const long long *)bsum, begX);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 2>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, startX, rowX, degX,
+                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, startX, degX,
                        (unsigned long long *)nullptr);
+    {
+        const int row_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
+        hipLaunchKernelGGL((side_rows_kernel<OffT>), dim3(row_blocks), dim3(256), 0, stream,
+                           (const OffT *)begX, V, rowX);
+    }
     return hipGetLastError();
 }
 
@@ -1247,7 +1289,7 @@ hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
     const int pass_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 0>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, (int *)nullptr, (int *)nullptr, (const OffT *)nullptr,
-                       (OffT *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr, stats);
+                       (OffT *)nullptr, (int32_t *)nullptr, stats);
     return hipGetLastError();
 }
 
