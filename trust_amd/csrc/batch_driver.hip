@@ -37,8 +37,8 @@ static std::vector<uintptr_t> capture_key(const Ctx *ctx, const trust_params &p)
                           ctx->feat.p,       ctx->cost.p,       ctx->tiers.p,
                           ctx->mlp_params.p, ctx->batch_vbase.p, ctx->batch_page.p,
                           ctx->batch_out.p,  ctx->side_cnt.p,   ctx->side_bsum.p,
-                          ctx->side_beg.p,   ctx->side_adj.p,   ctx->side_row.p,
-                          ctx->side_deg.p,   ctx->walk_acc.p,   ctx->seg_beg.p,
+                          ctx->side_beg.p,   ctx->side_ent.p,   ctx->side_row.p,
+                          ctx->walk_acc.p,   ctx->seg_beg.p,
                           ctx->seg_vbase.p,  ctx->seg_page.p};
     for (const void *q : ptrs)
         k.push_back((uintptr_t)q);

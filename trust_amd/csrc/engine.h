@@ -48,9 +48,9 @@ struct CsrViews {
     const int32_t *adjW;
     const void *begP;
     const int32_t *adjP;
-    // walk entries as list start (OffT) and length (null: adjW holds ids)
-    const void *startW = nullptr;
-    const int32_t *degW = nullptr;
+    // walk entries as WalkEntry<OffT> (list start and length; null: adjW
+    // holds ids)
+    const void *entW = nullptr;
 };
 
 // Shape of one counting launch.
@@ -114,7 +114,7 @@ struct Ctx {
     DevBuf staging, list, block_counts, ctl, spill, instr_cycles, instr_probes;
     // short-side walk: probe totals of both walks and the walk rows, a
     // per-graph index made the first time a count needs it
-    DevBuf side_cnt, side_bsum, side_beg, side_adj, side_row, side_deg, side_stats;
+    DevBuf side_cnt, side_bsum, side_beg, side_ent, side_row, side_stats;
     bool side_stats_ready = false, side_built = false;
     uint64_t side_w_ref = 0, side_w_short = 0;
     int64_t side_entries = 0;  // entries of the walk rows
@@ -123,7 +123,7 @@ struct Ctx {
     DevBuf walk_acc;  // per-vertex sum and max of the short-side walk's list lengths
     // start index of the reference walk (single graphs, int32 offsets): per
     // entry of adj, its list start and length
-    DevBuf ref_start, ref_deg;
+    DevBuf ref_ent;
     bool ref_built = false;
     bool mlp_ready = false;
     // the degree split as judge: which schedule counted the resident graph

@@ -223,8 +223,7 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.ctl = ctl_dev;
     a.instr_cycles = instr_cycles;
     a.instr_probes = instr_probes;
-    a.startW = (const OffT *)views.startW;
-    a.degW = views.degW;
+    a.entW = (const WalkEntry<OffT> *)views.entW;
     if (ctx->batch_ngraphs > 0) {  // read by the batch mode only
         const int64_t ng = ctx->batch_ngraphs;
         a.graph_vbase = (const long long *)ctx->batch_vbase.p;
@@ -420,11 +419,9 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
         if (!rc)
             rc = ctx->side_beg.ensure(ob * (size_t)(g.V + 1));
         if (!rc)
-            rc = ctx->side_adj.ensure(ob * (size_t)g.E);  // list starts
+            rc = ctx->side_ent.ensure(2 * ob * (size_t)g.E);  // WalkEntry: start, length
         if (!rc)
             rc = ctx->side_row.ensure(sizeof(int32_t) * (size_t)g.E);
-        if (!rc)
-            rc = ctx->side_deg.ensure(sizeof(int32_t) * (size_t)g.E);
         if (rc)
             return rc;
         if (g.off64)
@@ -432,17 +429,15 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
                                               g.V, (int *)ctx->side_cnt.p,
                                               (long long *)ctx->side_bsum.p,
                                               (int64_t *)ctx->side_beg.p,
-                                              (int64_t *)ctx->side_adj.p,
-                                              (int32_t *)ctx->side_row.p,
-                                              (int32_t *)ctx->side_deg.p, ctx->stream));
+                                              (WalkEntry<int64_t> *)ctx->side_ent.p,
+                                              (int32_t *)ctx->side_row.p, ctx->stream));
         else
             HIPCHK(launch_side_build<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
                                               g.V, (int *)ctx->side_cnt.p,
                                               (long long *)ctx->side_bsum.p,
                                               (int32_t *)ctx->side_beg.p,
-                                              (int32_t *)ctx->side_adj.p,
-                                              (int32_t *)ctx->side_row.p,
-                                              (int32_t *)ctx->side_deg.p, ctx->stream));
+                                              (WalkEntry<int32_t> *)ctx->side_ent.p,
+                                              (int32_t *)ctx->side_row.p, ctx->stream));
         if (g.off64) {
             int64_t n = 0;
             HIPCHK(hipMemcpyAsync(&n, (const int64_t *)ctx->side_beg.p + g.V, sizeof(n),
@@ -540,14 +535,11 @@ int ensure_start_index(Ctx *ctx, const ResidentRun &R)
         return TRUST_OK;
     if (g.off64 || R.segmented)
         return fail(TRUST_ERR_ARG, "start index of an int64 layout");
-    int rc = ctx->ref_start.ensure(sizeof(int32_t) * (size_t)g.E);
-    if (!rc)
-        rc = ctx->ref_deg.ensure(sizeof(int32_t) * (size_t)g.E);
+    int rc = ctx->ref_ent.ensure(sizeof(WalkEntry<int32_t>) * (size_t)g.E);
     if (rc)
         return rc;
     HIPCHK(launch_start_index<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p, g.E,
-                                       (int32_t *)ctx->ref_start.p, (int32_t *)ctx->ref_deg.p,
-                                       ctx->stream));
+                                       (WalkEntry<int32_t> *)ctx->ref_ent.p, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->ref_built = true;
     return TRUST_OK;
@@ -577,11 +569,9 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
     if (R.side) {  // walk rows built by prepare_side; hash and probe rows stay N+
         v.begW = ctx->side_beg.p;
         v.adjW = nullptr;
-        v.startW = ctx->side_adj.p;
-        v.degW = (const int32_t *)ctx->side_deg.p;
+        v.entW = ctx->side_ent.p;
     } else if (R.starts) {  // walk rows = N+, each entry's list read from the start index
-        v.startW = ctx->ref_start.p;
-        v.degW = (const int32_t *)ctx->ref_deg.p;
+        v.entW = ctx->ref_ent.p;
     }
     if (R.learned) {
         rc = enqueue_learned_plan(ctx, p, R.L, (TcCtl *)ctx->ctl.p, ctx->stream,

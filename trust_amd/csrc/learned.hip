@@ -20,6 +20,8 @@
 // result reports which schedule ran.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -151,7 +153,7 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
 // wsum / wmax (cleared before).
 __global__ void __launch_bounds__(256) walk_feat_kernel(const int32_t *__restrict__ rowX,
                                                         const int32_t *__restrict__ degX,
-                                                        long long X,
+                                                        int dstride, long long X,
                                                         unsigned long long *__restrict__ wsum,
                                                         unsigned int *__restrict__ wmax)
 {
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(256) walk_feat_kernel(const int32_t *__restric
         const long long pos = w * 64 + lane;
         const bool valid = pos < X;
         const int r = valid ? rowX[pos] : -1;
-        const int dv = valid ? degX[pos] : 0;
+        const int dv = valid ? degX[pos * dstride] : 0;
         const int rprev = __shfl_up(r, 1);
         const int rnext = __shfl_down(r, 1);
         const bool head = valid && (lane == 0 || rprev != r);
@@ -473,10 +475,13 @@ static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s, bool walk)
     HIPCHK(launch_fill_u32((uint32_t *)ctx->walk_acc.p, 0u, 3 * V, s));
     const long long X = ctx->side_entries;
     if (X > 0) {
+        // the length field of every WalkEntry<OffT> of the walk index
+        const int32_t *deg = (const int32_t *)((const char *)ctx->side_ent.p +
+                                               offsetof(WalkEntry<OffT>, deg));
+        const int dstride = (int)(sizeof(WalkEntry<OffT>) / sizeof(int32_t));
         const int wb = (int)std::min<long long>((X + 255) / 256, (long long)ctx->num_cus * 64);
         hipLaunchKernelGGL(walk_feat_kernel, dim3(wb), dim3(256), 0, s,
-                           (const int32_t *)ctx->side_row.p, (const int32_t *)ctx->side_deg.p, X,
-                           wsum, wmax);
+                           (const int32_t *)ctx->side_row.p, deg, dstride, X, wsum, wmax);
     }
     hipLaunchKernelGGL(walk_feat_final_kernel<OffT>, dim3(blocks), dim3(256), 0, s, begD, V,
                        (const unsigned long long *)wsum, (const unsigned int *)wmax, feat);

@@ -50,6 +50,14 @@ struct TcCtl {
 // in N+(u) whose lists are probed) and the probe rows (N+(v)).  For an
 // unpartitioned graph all three are the same arrays; for a 2D tile job they
 // are tiles (i,j), (i,k) and (k,j).
+// A walk entry of an index: the start of the list it probes and that
+// list's length, read together with one load and written with one store.
+template <typename OffT>
+struct alignas(2 * sizeof(OffT)) WalkEntry {
+    OffT start;
+    int32_t deg;
+};
+
 template <typename OffT>
 struct TcArgs {
     const OffT *begH;
@@ -88,9 +96,8 @@ struct TcArgs {
     unsigned long long *instr_cycles;  // per vertex (instrumented kernel only)
     unsigned long long *instr_probes;
     // Walk entries stored as the start and length of the list they probe
-    // (nullable; the short-side walk index).  Then adjW is not read.
-    const OffT *startW;
-    const int32_t *degW;
+    // (nullable; a walk index).  Then adjW is not read.
+    const WalkEntry<OffT> *entW;
 };
 
 // Planner parameters for the degree split.
@@ -132,20 +139,20 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream
 // view of the three-view kernel): row x = the v in N+(x) with
 // d+(v) <= d+(x), then the u with u -> x and d+(u) < d+(x), rows with
 // d+ < 2 contributing nothing.  cnt: 2V ints of scratch; bsum:
-// side_scan_blocks(V) + 1 long longs; begX: V + 1 offsets; startX (OffT),
-// rowX, degX: up to E each (the start of the entry's list in adj, the row,
-// the list length d+).  Launches only (no host sync).
+// side_scan_blocks(V) + 1 long longs; begX: V + 1 offsets; entX, rowX: up
+// to E each (the start and length d+ of the entry's list in adj; the row).
+// Launches only (no host sync).
 constexpr int kSideScanPer = 8192;  // vertices per scan workgroup
 inline long long side_scan_blocks(long long V) { return (V + kSideScanPer - 1) / kSideScanPer; }
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begX, OffT *startX, int32_t *rowX,
-                             int32_t *degX, hipStream_t stream);
-// Start index of the reference walk: for e < E, startR[e] = begP[adj[e]]
-// and degR[e] = its list length.  Launches only.
+                             long long *bsum, OffT *begX, WalkEntry<OffT> *entX, int32_t *rowX,
+                             hipStream_t stream);
+// Start index of the reference walk: for e < E, entR[e] = {begP[adj[e]],
+// its list length}.  Launches only.
 template <typename OffT>
-hipError_t launch_start_index(const OffT *begP, const int32_t *adj, long long E, OffT *startR,
-                              int32_t *degR, hipStream_t stream);
+hipError_t launch_start_index(const OffT *begP, const int32_t *adj, long long E,
+                              WalkEntry<OffT> *entR, hipStream_t stream);
 // stats[0] = probes of the reference walk, stats[1] = of the short-side walk
 // (stats: 2 unsigned long longs, cleared here).
 template <typename OffT>

@@ -371,18 +371,19 @@ __device__ __forceinline__ Group<OffT> load_group(int v, bool valid, const OffT 
 }
 
 // Group of 64 walk entries starting at entry `at` of the walk view.  A walk
-// index that stores each entry's list start and length (startW, degW) is
-// read in place, coalesced; otherwise the entry is a vertex id whose offsets
+// index that stores each entry's list start and length (entW) is read in
+// place, coalesced, one load per entry; otherwise the entry is a vertex id whose offsets
 // are looked up.
 template <typename OffT>
 __device__ __forceinline__ Group<OffT> walk_group(const TcArgs<OffT> &a, OffT at, bool valid)
 {
-    if (a.startW) {
+    if (a.entW) {
         OffT s = 0;
         int deg = 0;
         if (valid) {
-            s = a.startW[at];
-            deg = a.degW[at];
+            const WalkEntry<OffT> x = a.entW[at];
+            s = x.start;
+            deg = x.deg;
         }
         return make_group(s, deg);
     }
@@ -725,7 +726,7 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
     for (int c = 0; c < dW; c += kWave) {
         const bool valid = c + lane < dW;
         Group<OffT> g;
-        if (a.startW) {
+        if (a.entW) {
             g = walk_group(a, sW + (OffT)(c + lane), valid);
         } else {
             const int v = c == 0 && same_row ? head : (valid ? a.adjW[sW + (OffT)(c + lane)] : 0);
@@ -1003,8 +1004,8 @@ __global__ void offsets_to_i32_kernel(const int64_t *src, int32_t *dst, long lon
 //            each row belongs to one wave), incnt[v] += 1 per in-entry;
 //   kPass 2: writes the rows: out-entries at begX[u] + (LDS slot), in-entries
 //            from the row's end down (incnt counts back to zero).  An entry x
-//            is stored as its list's start and length (startX = beg[x],
-//            degX = d+(x)), which the count reads in place of x.
+//            is stored as its list's start and length (entX = {beg[x],
+//            d+(x)}, one store), which the count reads in place of x.
 // The entries' rows (rowX, the learned plan's features) follow from begX
 // alone and are written afterwards, in order, by side_rows_kernel: in the
 // scatter pass they were a third of the in-entries' random writes.
@@ -1016,8 +1017,7 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                                                         long long V, int *__restrict__ incnt,
                                                         int *__restrict__ kept,
                                                         const OffT *__restrict__ begX,
-                                                        OffT *__restrict__ startX,
-                                                        int32_t *__restrict__ degX,
+                                                        WalkEntry<OffT> *__restrict__ entX,
                                                         unsigned long long *__restrict__ stats)
 {
     unsigned long long w_ref = 0, w_short = 0;
@@ -1080,13 +1080,11 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
                         if (dv <= du) {
                             const int slot = atomicAdd(rowcnt + own[k], 1);
                             const OffT at = begX[u] + (OffT)slot;
-                            startX[at] = bv;
-                            degX[at] = dv;
+                            entX[at] = WalkEntry<OffT>{bv, dv};
                         } else {
                             const int k2 = atomicSub(incnt + v[k], 1);
                             const OffT at = begX[v[k] + 1] - (OffT)k2;
-                            startX[at] = su;
-                            degX[at] = du;
+                            entX[at] = WalkEntry<OffT>{su, du};
                         }
                     }
                 }
@@ -1241,8 +1239,8 @@ __global__ void __launch_bounds__(256) side_rows_kernel(const OffT *__restrict__
 
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begX, OffT *startX, int32_t *rowX,
-                             int32_t *degX, hipStream_t stream)
+                             long long *bsum, OffT *begX, WalkEntry<OffT> *entX, int32_t *rowX,
+                             hipStream_t stream)
 {
     if (V <= 0)
         return hipSuccess;
@@ -1253,8 +1251,8 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL((side_pass_kernel<OffT, 1>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, incnt, kept, (const OffT *)nullptr, (OffT *)nullptr,
-                       (int32_t *)nullptr, (unsigned long long *)nullptr);
+                       beg, adj, V, incnt, kept, (const OffT *)nullptr,
+                       (WalkEntry<OffT> *)nullptr, (unsigned long long *)nullptr);
     hipLaunchKernelGGL(side_scan_reduce_kernel, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, bsum);
     hipLaunchKernelGGL(side_scan_top_kernel<OffT>, dim3(1), dim3(1024), 0, stream, bsum, nb, V,
@@ -1262,7 +1260,7 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
     hipLaunchKernelGGL(side_scan_out_kernel<OffT>, dim3((unsigned)nb), dim3(1024), 0, stream,
                        (const int *)incnt, (const int *)kept, V, (const long long *)bsum, begX);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 2>), dim3(pass_blocks), dim3(256), 0, stream,
-                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, startX, degX,
+                       beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, entX,
                        (unsigned long long *)nullptr);
     {
         const int row_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
@@ -1273,10 +1271,10 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
 }
 
 template hipError_t launch_side_build<int32_t>(const int32_t *, const int32_t *, long long, int *,
-                                               long long *, int32_t *, int32_t *, int32_t *,
+                                               long long *, int32_t *, WalkEntry<int32_t> *,
                                                int32_t *, hipStream_t);
 template hipError_t launch_side_build<int64_t>(const int64_t *, const int32_t *, long long, int *,
-                                               long long *, int64_t *, int64_t *, int32_t *,
+                                               long long *, int64_t *, WalkEntry<int64_t> *,
                                                int32_t *, hipStream_t);
 
 template <typename OffT>
@@ -1289,41 +1287,40 @@ hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,
     const int pass_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
     hipLaunchKernelGGL((side_pass_kernel<OffT, 0>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, (int *)nullptr, (int *)nullptr, (const OffT *)nullptr,
-                       (OffT *)nullptr, (int32_t *)nullptr, stats);
+                       (WalkEntry<OffT> *)nullptr, stats);
     return hipGetLastError();
 }
 
 template <typename OffT>
 __global__ void __launch_bounds__(256) start_index_kernel(const OffT *__restrict__ begP,
                                                           const int32_t *__restrict__ adj,
-                                                          long long E, OffT *__restrict__ startR,
-                                                          int32_t *__restrict__ degR)
+                                                          long long E,
+                                                          WalkEntry<OffT> *__restrict__ entR)
 {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
         const int v = adj[e];
         const OffT s = begP[v];
-        startR[e] = s;
-        degR[e] = (int32_t)(begP[v + 1] - s);
+        entR[e] = WalkEntry<OffT>{s, (int32_t)(begP[v + 1] - s)};
     }
 }
 
 template <typename OffT>
-hipError_t launch_start_index(const OffT *begP, const int32_t *adj, long long E, OffT *startR,
-                              int32_t *degR, hipStream_t stream)
+hipError_t launch_start_index(const OffT *begP, const int32_t *adj, long long E,
+                              WalkEntry<OffT> *entR, hipStream_t stream)
 {
     if (E <= 0)
         return hipSuccess;
     const int blocks = (int)min((E + 255) / 256, 32768ll);
     hipLaunchKernelGGL((start_index_kernel<OffT>), dim3(blocks), dim3(256), 0, stream, begP, adj, E,
-                       startR, degR);
+                       entR);
     return hipGetLastError();
 }
 
 template hipError_t launch_start_index<int32_t>(const int32_t *, const int32_t *, long long,
-                                                int32_t *, int32_t *, hipStream_t);
+                                                WalkEntry<int32_t> *, hipStream_t);
 template hipError_t launch_start_index<int64_t>(const int64_t *, const int32_t *, long long,
-                                                int64_t *, int32_t *, hipStream_t);
+                                                WalkEntry<int64_t> *, hipStream_t);
 
 template hipError_t launch_side_stats<int32_t>(const int32_t *, const int32_t *, long long,
                                                unsigned long long *, hipStream_t);
