@@ -215,12 +215,17 @@ def bench_rmat(args, T):
     log(f"walk: {'short-side' if walk else 'reference'}; one-time walk index {index_s * 1e3:.1f} ms")
     deg = run_steps(lambda: ctx.count(**base_kw, schedule=T.SCHED_DEGREE), args.warmup,
                     args.steps, "degree split", E)
-    rw_kw = dict(base_kw, schedule=T.SCHED_DEGREE, flags=T.FLAG_REFERENCE_WALK)
+    # the reference's walk: every v in N+(u), its offsets looked up per entry
+    ref_flags = T.FLAG_REFERENCE_WALK | T.FLAG_OFFSET_LOOKUP
+    rw_kw = dict(base_kw, schedule=T.SCHED_DEGREE, flags=ref_flags)
     ref_walk = run_steps(lambda: ctx.count(**rw_kw), 1, max(1, args.steps // 2),
                          "degree split, reference walk", E)
+    # the same walk reading the offsets from this build's start index
+    rs_kw = dict(base_kw, schedule=T.SCHED_DEGREE, flags=T.FLAG_REFERENCE_WALK)
+    ref_starts = run_steps(lambda: ctx.count(**rs_kw), 1, max(1, args.steps // 2),
+                           "degree split, reference walk with the start index", E)
     # the reference's own launch (README: 1024 x 1024, chunk 1) and walk
-    ref_kw = dict(schedule=T.SCHED_DEGREE, threads=1024, blocks=1024, chunk=1,
-                  flags=T.FLAG_REFERENCE_WALK)
+    ref_kw = dict(schedule=T.SCHED_DEGREE, threads=1024, blocks=1024, chunk=1, flags=ref_flags)
     ref_launch = run_steps(lambda: ctx.count(**ref_kw), 1, max(1, args.steps // 2),
                            "degree split, reference launch and walk", E)
     train = tune = forced = None
@@ -249,6 +254,7 @@ def bench_rmat(args, T):
         head = run_steps(lambda: ctx.count(**head_kw), args.warmup, args.steps, "degree (head)", E)
     if head["r"].triangles != deg["r"].triangles or ref_launch["r"].triangles != deg["r"].triangles \
             or ref_walk["r"].triangles != deg["r"].triangles \
+            or ref_starts["r"].triangles != deg["r"].triangles \
             or (forced and forced["r"].triangles != deg["r"].triangles):
         raise SystemExit("schedules disagree on the count")
     r0, dev_s, cnt_s = head["r"], head["dev_s"], head["cnt_s"]
@@ -276,6 +282,7 @@ def bench_rmat(args, T):
         "count_kernel_ms": cnt_s * 1e3, "plan_ms": (dev_s - cnt_s) * 1e3,
         "degree_split_gteps": E / deg["dev_s"] / 1e9,
         "degree_split_reference_walk_gteps": E / ref_walk["dev_s"] / 1e9,
+        "degree_split_reference_walk_start_index_gteps": E / ref_starts["dev_s"] / 1e9,
         "degree_split_reference_launch_and_walk_gteps": E / ref_launch["dev_s"] / 1e9,
         "speedup_vs_degree_split": deg["dev_s"] / dev_s,
         "speedup_vs_degree_split_reference_walk": ref_walk["dev_s"] / dev_s,
