@@ -276,3 +276,18 @@ def test_segments_offset_lookup_flag(sctx, batch):
     for kw in (dict(flags=T.FLAG_OFFSET_LOOKUP), dict(flags=T.FLAG_HIP_GRAPH | T.FLAG_OFFSET_LOOKUP)):
         counts, _, r = sctx.count_batch(**kw)
         assert np.array_equal(counts, want), kw
+
+
+def test_segments_beside_learned_schedule(sctx, batch):
+    # the learned plan is built over the whole int64 batch: its counts run the
+    # int64 kernel between segmented degree counts, eagerly and captured
+    beg, adj, vb, want = batch
+    assert sctx.batch_segments(int(np.diff(beg[vb]).max())) > 1
+    sctx.train_cost_model(epochs=3, max_samples=20000, seed=2)
+    for kw in (dict(schedule=T.SCHED_LEARNED), dict(schedule=T.SCHED_DEGREE),
+               dict(schedule=T.SCHED_LEARNED, flags=T.FLAG_HIP_GRAPH),
+               dict(schedule=T.SCHED_DEGREE, flags=T.FLAG_HIP_GRAPH),
+               dict(schedule=T.SCHED_LEARNED, flags=T.FLAG_HIP_GRAPH)):
+        counts, _, r = sctx.count_batch(**kw)
+        assert np.array_equal(counts, want), kw
+        assert r.schedule_used == kw["schedule"]
