@@ -58,3 +58,28 @@ def test_cpu_baseline_object_on_a_small_graph():
     assert line["kind"] == "port" and line["unit"] == "GTEPS"
     assert line["value"] > 0 and line["cores"] >= 1
     assert "oracle" in line["sample"]
+
+
+def test_profile_commands_run_committed_files_only():
+    # every summary under profiles/ names the command that made it; each runs
+    # bench.py (under rocprofv3 or alone) with options bench.py defines
+    import re
+
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    known = set(re.findall(r'add_argument\("(--[a-z0-9-]+)"', src))
+    prof = os.path.join(ROOT, "profiles")
+    for name in sorted(os.listdir(prof)):
+        cmds = []
+        for line in open(os.path.join(prof, name)):
+            line = line.strip().lstrip("#").strip()
+            if line.startswith(("python3 ", "rocprofv3 ")):
+                cmds.append(line)
+        assert cmds, f"{name} names no command"
+        for cmd in cmds:
+            assert "_work" not in cmd and "/root/" not in cmd, cmd
+            tail = cmd.split(" -- ", 1)[1] if cmd.startswith("rocprofv3") else cmd
+            words = tail.split()
+            assert words[0] == "python3" and "bench.py" in words, cmd
+            for w in words:
+                if w.startswith("--"):
+                    assert w in known, (name, w)
