@@ -128,14 +128,20 @@ typedef struct trust_arguments {
 typedef struct trust_ctx trust_ctx;
 
 /* ---- library ---------------------------------------------------------- */
+/* Message of this thread's last failed call.  The reference prints and exits
+ * on any CUDA error instead (HRR, Without-graph-partition/herror.h:1-8). */
 const char *trust_last_error(void);
 const char *trust_version(void);
-/* Defaults: threads 0, blocks 0, chunk 0, rank 0, total_process 1,
+/* The launch fields stand for the reference's argv[3..5] and the MPI rank /
+ * argv[2] (Without-graph-partition/mpi_main.cpp:78-81).
+ * Defaults: threads 0, blocks 0, chunk 0, rank 0, total_process 1,
  * schedule DEGREE, hub_threshold 100, wave_slots 0, lds_budget 0.  With
  * schedule LEARNED or AUTO and no cost model loaded, the degree split runs;
  * trust_result.schedule_used says which schedule ran. */
 void trust_params_init(trust_params *p);
-/* Number of HIP devices (0 when none); negative on runtime failure. */
+/* Number of HIP devices (0 when none); negative on runtime failure.  The
+ * reference picks device (rank + 1) % count itself (TC.cu:615-621); here the
+ * device is trust_params.device. */
 int trust_device_count(void);
 
 /* ---- drop-in counting entry points ------------------------------------ */
@@ -159,7 +165,9 @@ int trust_triangle_count_partition(int rank, const char *folder, int total_proce
                                    int threads, int blocks, int chunk_size,
                                    int partition_num, trust_arguments *out);
 
-/* Count over a host CSR (copied to the device inside the call). */
+/* Triangle_count over a CSR already in host memory instead of a folder
+ * (Without-graph-partition/TC.cu:602-731 without its file load); copied to the
+ * device inside the call. */
 int trust_count_csr(const int64_t *beg, const int32_t *adj, int64_t vertices, int64_t edges,
                     const trust_params *p, trust_result *r);
 
@@ -171,6 +179,10 @@ int trust_count_csr(const int64_t *beg, const int32_t *adj, int64_t vertices, in
 int trust_count_tiles(const trust_csr *tiles, int n, const trust_params *p, trust_result *r);
 
 /* ---- resident graphs -------------------------------------------------- */
+/* No counterpart in the reference, whose every Triangle_count call loads,
+ * copies and frees the graph (Without-graph-partition/TC.cu:611-711): a
+ * context keeps one resident on the device for repeated counts, and keeps
+ * its per-graph walk indexes. */
 int trust_ctx_create(int device, trust_ctx **out);
 void trust_ctx_destroy(trust_ctx *ctx);
 /* Copy a CSR to the device once; later counts reuse it.  h2d_seconds nullable. */
@@ -313,8 +325,11 @@ int trust_ctx_batch_segments(trust_ctx *ctx, int64_t max_edges, int64_t *segment
 /* Read <folder>/begin.bin + adjacent.bin (Without-graph-partition/graph.hpp:18-67)
  * with size and content validation. */
 int trust_csr_load(const char *folder, trust_csr *out);
+/* Write <folder>/begin.bin (int64 offsets) + adjacent.bin (int32 ids), the
+ * files Preprocess/preprocess_outdegree.cpp:205-222 writes. */
 int trust_csr_save(const char *folder, const trust_csr *csr);
-/* Check offsets (monotone, beg[0]=0, beg[V]=E) and ids (< V). */
+/* Check offsets (monotone, beg[0]=0, beg[V]=E) and ids (< V).  No
+ * counterpart: the reference reads the files unchecked (graph.hpp:18-67). */
 int trust_csr_validate(const int64_t *beg, const int32_t *adj, int64_t vertices, int64_t edges);
 void trust_csr_free(trust_csr *csr);
 
@@ -364,6 +379,9 @@ int trust_partition_write(const char *folder, const trust_csr *csr, int n);
 int trust_partition_folder(const char *folder, int n);
 
 /* ---- seeded synthetic generators (edge lists, not yet preprocessed) --- */
+/* No counterpart in the reference, which downloads its datasets
+ * (Dataset/Cit-Patents/get+preprocess.sh); the output goes through the same
+ * preprocessing as a downloaded edge list. */
 /* R-MAT: m = edge_factor << scale samples into src/dst (caller-allocated). */
 int trust_gen_rmat(int scale, int64_t m, double a, double b, double c, uint64_t seed,
                    int32_t *src, int32_t *dst);
