@@ -223,3 +223,19 @@ def test_fresh_inputs_match_reference_tools(kind, seed):
         for f in ("begin.bin", "adjacent.bin", "edge"):
             assert _bytes(os.path.join(r, f)) == _bytes(os.path.join(o, f)), (order, f)
         os.rename(os.path.join(r, "1.mmio"), os.path.join(ref, "1.mmio"))
+
+
+def test_txt2bin_many_blocks_against_restatement(tmp_path):
+    # more pairs than one storage block (2^22): ids compacted in ascending
+    # order, degrees and remapped pairs as TransformTxt2Binary.cpp:97-152 write them
+    rng = np.random.default_rng(5)
+    m = (1 << 22) + 1_000_003
+    ids = rng.integers(0, 3_000_000, 2 * m) * 3 + 7  # sparse ids, some never used
+    with open(tmp_path / "e.txt", "w") as f:
+        f.write("\n".join(f"{ids[2 * i]} {ids[2 * i + 1]}" for i in range(m)))
+    subprocess.run([TOOL, "txt2bin", "e.txt"], cwd=tmp_path, check=True, capture_output=True,
+                   timeout=300)
+    uniq, inv, cnt = np.unique(ids, return_inverse=True, return_counts=True)
+    assert (tmp_path / "prop.txt").read_text().split() == [str(m), str(len(uniq))]
+    assert np.array_equal(np.fromfile(tmp_path / "degree", np.int32), cnt.astype(np.int32))
+    assert np.array_equal(np.fromfile(tmp_path / "BinEdgelist", np.int32), inv.astype(np.int32))

@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <memory>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -154,11 +155,29 @@ static bool read_pairs(const std::string &path, std::vector<int32_t> &a, std::ve
     return true;
 }
 
+// Id pairs, interleaved (u, v), in fixed-size blocks: reading a large list
+// needs no reallocation copy (a growing vector holds its old and new buffers
+// at once), and the pairs are rewritten in place and written block by block.
+struct PairBlocks {
+    static constexpr size_t kBlock = (size_t)1 << 22;  // pairs per block (32 MiB)
+    std::vector<std::unique_ptr<int32_t[]>> blocks;
+    size_t n = 0;
+    void push(int32_t u, int32_t v)
+    {
+        if (n % kBlock == 0)
+            blocks.emplace_back(new int32_t[2 * kBlock]);
+        int32_t *p = blocks.back().get() + 2 * (n % kBlock);
+        p[0] = u;
+        p[1] = v;
+        n++;
+    }
+    size_t size_of(size_t k) const { return std::min(kBlock, n - k * kBlock); }  // pairs in block k
+};
+
 // Whitespace-separated integer pairs (the fscanf "%d%d" loop of
 // TransformTxt2Binary.cpp:85-95, which pairs tokens across line ends).  Ids
 // must lie in [0, 2^31).
-static bool read_int_pairs(const std::string &path, std::vector<int32_t> &a,
-                           std::vector<int32_t> &b, std::string &err)
+static bool read_int_pairs(const std::string &path, PairBlocks &pb, std::string &err)
 {
     ByteReader r(path);
     if (!r.ok()) {
@@ -187,8 +206,7 @@ static bool read_int_pairs(const std::string &path, std::vector<int32_t> &a,
                 u = (int32_t)x;
                 have_u = true;
             } else {
-                a.push_back(u);
-                b.push_back((int32_t)x);
+                pb.push(u, (int32_t)x);
                 have_u = false;
             }
             continue;
@@ -215,28 +233,30 @@ static int write_bytes(const std::string &path, const void *p, size_t n)
     return fclose(f) == 0 ? 0 : 1;
 }
 
+// The edge_count_hint argument is accepted for the reference's argv and not
+// needed: the pairs are read into blocks.
 static int cmd_txt2bin(int argc, char **argv)
 {
     if (argc < 1)
         return 2;
-    std::vector<int32_t> a, b;
-    if (argc > 1 && atoll(argv[1]) > 0) {
-        a.reserve((size_t)atoll(argv[1]));
-        b.reserve((size_t)atoll(argv[1]));
-    }
+    PairBlocks pb;
     std::string err;
-    if (!read_int_pairs(argv[0], a, b, err)) {
+    if (!read_int_pairs(argv[0], pb, err)) {
         fprintf(stderr, "trust_tool: %s: %s\n", argv[0], err.c_str());
         return 1;
     }
     // ids that occur, compacted in ascending order (TransformTxt2Binary.cpp:97-121)
     int64_t maxid = -1;
-    for (size_t e = 0; e < a.size(); e++)
-        maxid = std::max<int64_t>(maxid, std::max(a[e], b[e]));
+    for (size_t k = 0; k < pb.blocks.size(); k++) {
+        const int32_t *p = pb.blocks[k].get();
+        for (size_t i = 0; i < 2 * pb.size_of(k); i++)
+            maxid = std::max<int64_t>(maxid, p[i]);
+    }
     std::vector<int64_t> occ((size_t)(maxid + 1), 0);
-    for (size_t e = 0; e < a.size(); e++) {
-        occ[a[e]]++;
-        occ[b[e]]++;
+    for (size_t k = 0; k < pb.blocks.size(); k++) {
+        const int32_t *p = pb.blocks[k].get();
+        for (size_t i = 0; i < 2 * pb.size_of(k); i++)
+            occ[p[i]]++;
     }
     std::vector<int32_t> degree;
     std::vector<int32_t> cid((size_t)(maxid + 1), -1);
@@ -251,18 +271,29 @@ static int cmd_txt2bin(int argc, char **argv)
             degree.push_back((int32_t)occ[x]);
         }
     std::vector<int64_t>().swap(occ);
-    std::vector<int32_t> pairs(2 * a.size());
-    for (size_t e = 0; e < a.size(); e++) {
-        pairs[2 * e] = cid[a[e]];
-        pairs[2 * e + 1] = cid[b[e]];
-    }
     char prop[64];
-    const int np = snprintf(prop, sizeof prop, "%zu %zu", a.size(), degree.size());
+    const int np = snprintf(prop, sizeof prop, "%zu %zu", pb.n, degree.size());
     if (write_bytes("prop.txt", prop, (size_t)np) ||
-        write_bytes("degree", degree.data(), degree.size() * sizeof(int32_t)) ||
-        write_bytes("BinEdgelist", pairs.data(), pairs.size() * sizeof(int32_t)))
+        write_bytes("degree", degree.data(), degree.size() * sizeof(int32_t)))
         return 1;
-    printf("pairs=%zu vertices=%zu\n", a.size(), degree.size());
+    // BinEdgelist: the compacted pairs, rewritten in place and written block by block
+    FILE *f = fopen("BinEdgelist", "wb");
+    bool ok = f != nullptr;
+    for (size_t k = 0; ok && k < pb.blocks.size(); k++) {
+        int32_t *p = pb.blocks[k].get();
+        const size_t m = 2 * pb.size_of(k);
+        for (size_t i = 0; i < m; i++)
+            p[i] = cid[p[i]];
+        ok = fwrite(p, sizeof(int32_t), m, f) == m;
+        pb.blocks[k].reset();
+    }
+    if (f)
+        ok = fclose(f) == 0 && ok;
+    if (!ok) {
+        fprintf(stderr, "trust_tool: cannot write BinEdgelist\n");
+        return 1;
+    }
+    printf("pairs=%zu vertices=%zu\n", pb.n, degree.size());
     return 0;
 }
 
