@@ -239,3 +239,23 @@ def test_txt2bin_many_blocks_against_restatement(tmp_path):
     assert (tmp_path / "prop.txt").read_text().split() == [str(m), str(len(uniq))]
     assert np.array_equal(np.fromfile(tmp_path / "degree", np.int32), cnt.astype(np.int32))
     assert np.array_equal(np.fromfile(tmp_path / "BinEdgelist", np.int32), inv.astype(np.int32))
+
+
+def test_undirect_many_blocks_against_restatement(tmp_path):
+    # more pairs than one storage block (2^22), with repeats in both directions
+    # and self-loops: 1.mmio = "k k m", then the unique pairs (min, max) of the
+    # 1-based compacted ids in ascending order
+    rng = np.random.default_rng(6)
+    m = (1 << 22) + 777_777
+    ids = rng.integers(0, 400_000, 2 * m) * 2 + 1
+    with open(tmp_path / "e.txt", "w") as f:
+        f.write("\n".join(f"{ids[2 * i]} {ids[2 * i + 1]}" for i in range(m)))
+    subprocess.run([TOOL, "undirect", "e.txt", "1.mmio"], cwd=tmp_path, check=True,
+                   capture_output=True, timeout=300)
+    uniq, inv = np.unique(ids, return_inverse=True)
+    p = (inv + 1).reshape(-1, 2)
+    p = np.unique(np.stack([p.min(axis=1), p.max(axis=1)], axis=1), axis=0)
+    lines = (tmp_path / "1.mmio").read_text().split("\n")
+    assert lines[0].split() == [str(len(uniq)), str(len(uniq)), str(len(p))]
+    got = np.array(" ".join(lines[1:]).split(), np.int64).reshape(-1, 2)
+    assert np.array_equal(got, p)

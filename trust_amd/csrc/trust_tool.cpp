@@ -104,13 +104,33 @@ static bool parse_int(ByteReader &r, int &c, long long &x)
     return x <= INT32_MAX;
 }
 
+// Id pairs, interleaved (u, v), in fixed-size blocks: reading a large list
+// needs no reallocation copy (a growing vector holds its old and new buffers
+// at once), and the pairs are rewritten in place and written block by block.
+struct PairBlocks {
+    static constexpr size_t kBlock = (size_t)1 << 22;  // pairs per block (32 MiB)
+    std::vector<std::unique_ptr<int32_t[]>> blocks;
+    size_t n = 0;
+    void push(int32_t u, int32_t v)
+    {
+        if (n % kBlock == 0)
+            blocks.emplace_back(new int32_t[2 * kBlock]);
+        int32_t *p = blocks.back().get() + 2 * (n % kBlock);
+        p[0] = u;
+        p[1] = v;
+        n++;
+    }
+    size_t size_of(size_t k) const { return std::min(kBlock, n - k * kBlock); }  // pairs in block k
+};
+
 // Line-oriented edge text as the reference's readers take it
 // (fromDirectToUndirect.cpp:36-52, preprocess_outdegree.cpp:49-78): lines
 // that do not start with a digit are skipped, a line gives the first two
 // integers on it (a third, the 1.mmio weight column, is ignored), and with
-// `header` the first such line is "rows cols entries" (hv[0], hv[1]).
-static bool read_pairs(const std::string &path, std::vector<int32_t> &a, std::vector<int32_t> &b,
-                       bool header, long long *hv)
+// `header` the first such line is "rows cols entries" (hv[0], hv[1]).  Each
+// pair goes to push(u, v).
+template <typename Push>
+static bool read_pairs(const std::string &path, Push &&push, bool header, long long *hv)
 {
     ByteReader r(path);
     if (!r.ok())
@@ -148,31 +168,11 @@ static bool read_pairs(const std::string &path, std::vector<int32_t> &a, std::ve
             hv[1] = v[1];
             first = false;
         } else if (nv >= 2) {
-            a.push_back((int32_t)v[0]);
-            b.push_back((int32_t)v[1]);
+            push((int32_t)v[0], (int32_t)v[1]);
         }
     }
     return true;
 }
-
-// Id pairs, interleaved (u, v), in fixed-size blocks: reading a large list
-// needs no reallocation copy (a growing vector holds its old and new buffers
-// at once), and the pairs are rewritten in place and written block by block.
-struct PairBlocks {
-    static constexpr size_t kBlock = (size_t)1 << 22;  // pairs per block (32 MiB)
-    std::vector<std::unique_ptr<int32_t[]>> blocks;
-    size_t n = 0;
-    void push(int32_t u, int32_t v)
-    {
-        if (n % kBlock == 0)
-            blocks.emplace_back(new int32_t[2 * kBlock]);
-        int32_t *p = blocks.back().get() + 2 * (n % kBlock);
-        p[0] = u;
-        p[1] = v;
-        n++;
-    }
-    size_t size_of(size_t k) const { return std::min(kBlock, n - k * kBlock); }  // pairs in block k
-};
 
 // Whitespace-separated integer pairs (the fscanf "%d%d" loop of
 // TransformTxt2Binary.cpp:85-95, which pairs tokens across line ends).  Ids
@@ -340,28 +340,44 @@ static int cmd_undirect(int argc, char **argv)
     if (argc < 1)
         return 2;
     std::string out = argc > 1 ? argv[1] : "1.mmio";
-    std::vector<int32_t> a, b;
-    if (!read_pairs(argv[0], a, b, false, nullptr)) {
+    PairBlocks pb;
+    if (!read_pairs(argv[0], [&](int32_t u, int32_t v) { pb.push(u, v); }, false, nullptr)) {
         fprintf(stderr, "trust_tool: cannot read %s\n", argv[0]);
         return 1;
     }
     int32_t maxid = -1;
-    for (size_t e = 0; e < a.size(); e++)
-        maxid = std::max(maxid, std::max(a[e], b[e]));
+    for (size_t q = 0; q < pb.blocks.size(); q++) {
+        const int32_t *p = pb.blocks[q].get();
+        for (size_t i = 0; i < 2 * pb.size_of(q); i++)
+            maxid = std::max(maxid, p[i]);
+    }
     std::vector<int32_t> id((size_t)maxid + 1, 0);
-    for (size_t e = 0; e < a.size(); e++)
-        id[a[e]] = id[b[e]] = 1;
+    for (size_t q = 0; q < pb.blocks.size(); q++) {
+        const int32_t *p = pb.blocks[q].get();
+        for (size_t i = 0; i < 2 * pb.size_of(q); i++)
+            id[p[i]] = 1;
+    }
     int64_t k = 0;
     for (auto &x : id)
         x = x ? (int32_t)++k : 0;  // 1-based
-    std::vector<uint64_t> keys(a.size()), tmp(a.size());
-    for (size_t e = 0; e < a.size(); e++) {
-        uint64_t u = (uint64_t)id[a[e]], v = (uint64_t)id[b[e]];
-        if (u > v)
-            std::swap(u, v);
-        keys[e] = (u << 32) | v;
+    // keys (u < v) block by block, each block freed once converted; the sort's
+    // scratch is allocated after: 16 bytes per pair at the peak
+    std::vector<uint64_t> keys(pb.n);
+    size_t e = 0;
+    for (size_t q = 0; q < pb.blocks.size(); q++) {
+        const int32_t *p = pb.blocks[q].get();
+        for (size_t i = 0; i < pb.size_of(q); i++, e++) {
+            uint64_t u = (uint64_t)id[p[2 * i]], v = (uint64_t)id[p[2 * i + 1]];
+            if (u > v)
+                std::swap(u, v);
+            keys[e] = (u << 32) | v;
+        }
+        pb.blocks[q].reset();
     }
+    std::vector<int32_t>().swap(id);
+    std::vector<uint64_t> tmp(keys.size());
     radix_sort_u64(keys.data(), tmp.data(), keys.size(), 64);
+    std::vector<uint64_t>().swap(tmp);
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     FILE *f = fopen(out.c_str(), "w");
     if (!f) {
@@ -450,7 +466,14 @@ static int cmd_preprocess(int argc, char **argv)
     std::string outdir = pos.size() > 1 ? pos[1] : ".";
     std::vector<int32_t> a, b;
     long long hv[2] = {-1, -1};
-    if (!read_pairs(in, a, b, true, hv) || hv[0] < 0 || hv[1] < 0) {
+    if (!read_pairs(
+            in,
+            [&](int32_t u, int32_t v) {
+                a.push_back(u);
+                b.push_back(v);
+            },
+            true, hv) ||
+        hv[0] < 0 || hv[1] < 0) {
         fprintf(stderr, "trust_tool: cannot read %s\n", in.c_str());
         return 1;
     }
