@@ -45,7 +45,8 @@ extern "C" {
  * Same triangles, min(d+(u), d+(v)) probes per edge.  It needs the in-walk
  * rows (for every v, the u -> v with 2 <= d+(u) < d+(v)), a per-graph index
  * that a context builds once, the first time a count uses it; that call
- * reports the build in trust_result.index_seconds, outside seconds.
+ * reports the build in trust_result.index_seconds and includes it in
+ * seconds.
  * Without either flag an unpartitioned count takes the short-side walk when
  * its probes are at most 45% of the reference walk's (a pass over the graph,
  * also once per graph and reported the same way); partitioned counts always
@@ -56,9 +57,22 @@ extern "C" {
  * each walked v's list start and length from a per-graph start index
  * (start[e] = beg[adj[e]], len[e] = d+(adj[e]), read in order with the walk)
  * instead of looking up beg[v] and beg[v + 1] at random; built once, the
- * first time a count uses it, and reported in trust_result.index_seconds.
- * Batches and int64 offsets keep the lookups.  This flag keeps them too. */
+ * first time a count uses it, and reported in trust_result.index_seconds
+ * (included in seconds).  Batches and int64 offsets keep the lookups.  This
+ * flag keeps them too. */
 #define TRUST_FLAG_OFFSET_LOOKUP 512u
+/* Discard the resident graph's per-graph indexes (walk statistics, the
+ * short-side walk rows, the start index) before counting, so that this
+ * count makes the ones it uses again, inside its seconds: the cost of
+ * counting a freshly loaded graph, as the reference's timed region covers
+ * all device work of its count (Without-graph-partition/TC.cu:686-692),
+ * without reloading the graph. */
+#define TRUST_FLAG_FRESH_INDEX 1024u
+/* Test aid: every hash table is built as a cuckoo table first and then
+ * treated as if an insertion had failed, i.e. cleared and rebuilt with
+ * linear probing -- the path a real failure takes (counted in
+ * trust_result.linear_rebuilds).  Counts are unchanged. */
+#define TRUST_FLAG_REBUILD_LINEAR 2048u
 /* Any other bit in trust_params.flags is rejected with TRUST_ERR_ARG. */
 
 /*
@@ -89,7 +103,8 @@ typedef struct trust_result {
     int64_t edges;         /* E (entries of adjacent.bin = undirected edges) */
     uint64_t triangles;    /* exact count (this rank's share) */
     uint64_t probes;       /* W: hash lookups performed by this call */
-    double seconds;        /* device time, planning + counting (hipEvents) */
+    double seconds;        /* device time of this call's work (hipEvents): per-graph index
+                              work it did (index_seconds) + planning + counting */
     double count_seconds;  /* device time of the counting launch alone */
     double plan_seconds;   /* device time of the planning kernels */
     double h2d_seconds;    /* host-to-device copy time (not part of seconds) */
@@ -104,8 +119,11 @@ typedef struct trust_result {
     int32_t wave_slots;    /* LDS table slots per wavefront */
     int32_t schedule_used; /* TRUST_SCHED_* that ran (LEARNED falls back to DEGREE without a model) */
     int32_t walk_used;     /* 0: the reference walk, 1: the short-side walk */
-    double index_seconds;  /* one-time per-graph work this call did before its timed region:
-                              walk statistics and the in-walk rows (0 once built) */
+    double index_seconds;  /* device time of the per-graph index work this call did (walk
+                              statistics, the in-walk rows or the start index; 0 when a
+                              context reuses them), included in seconds */
+    uint64_t linear_rebuilds; /* hash tables rebuilt with linear probing after a cuckoo
+                                 insertion failed (or was forced to, TRUST_FLAG_REBUILD_LINEAR) */
 } trust_result;
 
 /* Library-allocated CSR (trust_csr_free releases it). */
@@ -120,7 +138,7 @@ typedef struct trust_csr {
 typedef struct trust_arguments {
     int64_t edge_count; /* reference: int */
     uint64_t count;     /* reference: long long */
-    double time;        /* seconds of the counting launch */
+    double time;        /* device seconds of the count: index work, planning and counting */
     int64_t degree;     /* edge_count / vertices */
     int64_t vertices;   /* reference: int */
 } trust_arguments;

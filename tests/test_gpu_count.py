@@ -114,12 +114,32 @@ def test_short_side_walk_saves_probes(ctx16, rmat16):
     assert again.index_seconds == 0.0  # the in-walk rows are built once per graph
 
 
+@pytest.mark.parametrize("walk", [T.FLAG_SHORT_WALK, T.FLAG_REFERENCE_WALK, 0])
+def test_fresh_index_is_timed(ctx16, rmat16, walk):
+    # FLAG_FRESH_INDEX rebuilds the walk's per-graph index inside the count's
+    # seconds (index + plan + count); without it a context reuses the index
+    beg, adj, want, _ = rmat16
+    kept = ctx16.count(flags=walk)
+    f1 = ctx16.count(flags=walk | T.FLAG_FRESH_INDEX)
+    f2 = ctx16.count(flags=walk | T.FLAG_FRESH_INDEX)
+    again = ctx16.count(flags=walk)
+    print(f"[gpu] rmat16 walk flag {walk}: fresh {f1.seconds * 1e3:.3f} ms (index "
+          f"{f1.index_seconds * 1e3:.3f} ms), kept {again.seconds * 1e3:.3f} ms", flush=True)
+    for r in (kept, f1, f2, again):
+        assert r.triangles == want
+        assert r.seconds == pytest.approx(r.index_seconds + r.plan_seconds + r.count_seconds,
+                                          rel=1e-9, abs=1e-12)
+    assert f1.index_seconds > 0 and f2.index_seconds > 0
+    assert again.index_seconds == 0
+    assert f1.walk_used == again.walk_used
+
+
 def test_walk_flags_together_rejected(ctx16):
     with pytest.raises(T.TrustError):
         ctx16.count(flags=T.FLAG_REFERENCE_WALK | T.FLAG_SHORT_WALK)
     with pytest.raises(T.TrustError):
         T.count_tiles(T.partition_csr(*T.rmat_csr(8, 8, seed=1), 2), 2,
-                      flags=T.FLAG_SHORT_WALK)
+                      flags=T.FLAG_REFERENCE_WALK | T.FLAG_SHORT_WALK)
 
 
 def test_default_walk_choice_rmat21(gpu):
@@ -143,7 +163,7 @@ def test_default_walk_choice_rmat21(gpu):
 
 
 def test_unknown_flags_rejected(ctx16):
-    for bad in (32, 1024, 1 << 31):
+    for bad in (32, 1 << 12, 1 << 31):
         with pytest.raises(T.TrustError):
             ctx16.count(flags=bad)
 
@@ -275,6 +295,11 @@ def test_drop_in_entry_point_and_cli(gpu, tmp_path, rmat16):
     T.save_folder(folder, beg, adj)
     a = T.triangle_count(0, folder, 1, 1024, 1024, 1)
     assert a.count == want and a.edge_count == len(adj) and a.vertices == len(beg) - 1
+    # a call on a fresh context builds its walk's index and times it with the count
+    r = T.count_csr(beg, adj, threads=1024, blocks=1024, chunk=1)
+    assert r.triangles == want and r.index_seconds > 0
+    assert r.seconds == pytest.approx(r.index_seconds + r.plan_seconds + r.count_seconds)
+    assert a.time > 0
     exe = os.path.join(T.BIN_DIR, "trianglecounting")
     out = subprocess.run([exe, folder, "3", "1024", "1024", "1"], capture_output=True, text=True,
                          check=True, timeout=120).stdout

@@ -259,3 +259,26 @@ def test_undirect_many_blocks_against_restatement(tmp_path):
     assert lines[0].split() == [str(len(uniq)), str(len(uniq)), str(len(p))]
     got = np.array(" ".join(lines[1:]).split(), np.int64).reshape(-1, 2)
     assert np.array_equal(got, p)
+
+
+@pytest.mark.parametrize("text", ["3 -5\n", "3 -2000000000\n", "1 2\n4 -7\n"])
+def test_undirect_rejects_negative_ids(tmp_path, text):
+    # the reference's fromDirectToUndirect indexes its arrays with the id
+    # (undefined behaviour); the tool must stop with an error and no output
+    (tmp_path / "e.txt").write_text(text)
+    r = subprocess.run([TOOL, "undirect", "e.txt", "1.mmio"], cwd=tmp_path, capture_output=True,
+                       timeout=60)
+    assert r.returncode != 0
+    assert b"negative vertex id" in r.stderr
+    assert not (tmp_path / "1.mmio").exists()
+
+
+@pytest.mark.parametrize("body", ["1 0 1\n", "0 2 1\n", "2 -3 1\n", "1 -2147483648 1\n"])
+def test_preprocess_rejects_ids_below_one(tmp_path, body):
+    # 1.mmio ids are 1-based (preprocess_outdegree.cpp:73-74 decrements them)
+    (tmp_path / "1.mmio").write_text("3 3 1\n" + body)
+    r = subprocess.run([TOOL, "preprocess", "1.mmio", "."], cwd=tmp_path, capture_output=True,
+                       timeout=60)
+    assert r.returncode != 0
+    assert b"ids start at 1" in r.stderr
+    assert not (tmp_path / "begin.bin").exists()

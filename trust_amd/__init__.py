@@ -33,6 +33,8 @@ FLAG_HIP_GRAPH = 64
 FLAG_REFERENCE_WALK = 128
 FLAG_SHORT_WALK = 256
 FLAG_OFFSET_LOOKUP = 512
+FLAG_FRESH_INDEX = 1024
+FLAG_REBUILD_LINEAR = 2048
 ORDER_OUTDEGREE, ORDER_PLAIN, ORDER_INDEGREE = 0, 1, 2
 ORDERS = {"outdegree": ORDER_OUTDEGREE, "plain": ORDER_PLAIN, "indegree": ORDER_INDEGREE}
 PRE_SELF_LOOP_ONCE = 1
@@ -65,6 +67,7 @@ class Result(C.Structure):
         ("hub_slots", C.c_int32), ("wave_slots", C.c_int32),
         ("schedule_used", C.c_int32), ("walk_used", C.c_int32),
         ("index_seconds", C.c_double),
+        ("linear_rebuilds", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -189,6 +189,7 @@ int read_ctl(Ctx *ctx, const ResidentRun &R, TcCtl &h)
     for (size_t k = 1; k < n; k++) {  // list_begin is summed too: tier sizes are its differences
         h.count += hs[k].count;
         h.probes += hs[k].probes;
+        h.rebuilds += hs[k].rebuilds;
         for (int t = 0; t < 8; t++)
             h.list_begin[t] += hs[k].list_begin[t];
     }
@@ -353,6 +354,5 @@ extern "C" int trust_ctx_count_batch(trust_ctx *c, const trust_params *p, uint64
             done_seconds[g] = d > t0 ? (double)(d - t0) / ctx->wall_hz : 0.0;
         }
     }
-    finish_resident(ctx, R, h, t_plan * 1e-3, t_count * 1e-3, r);
-    return TRUST_OK;
+    return finish_resident(ctx, R, h, t_plan * 1e-3, t_count * 1e-3, r);
 }

@@ -97,6 +97,47 @@ struct SegView {
     unsigned long long *graph_counts, *graph_done;
 };
 
+// A slice [lo, hi) of one target bin's in-entry records (walk_index.hip);
+// multi >= 0 numbers the slices of bins cut into more than one.
+struct WiSlice {
+    int bin;
+    int multi;
+    long long lo, hi;
+};
+
+// C2's work queue of the binned walk-index build (walk_index.hip): chunks
+// of a slice's records, in one list per XCD.
+constexpr int kXcds = 8;
+struct WiChunk {
+    int slice;
+    int pad;
+    long long lo, hi;
+};
+struct WiQueue {
+    int begin[kXcds + 1];           // XCD x's chunks are [begin[x], begin[x+1])
+    unsigned int next[kXcds * 32];  // cursor of XCD x at next[32 x] (its own 128-byte line)
+};
+
+// Scratch and host-side state of the binned walk-index build.
+struct WalkIndexState {
+    DevBuf M, O, bin_start, bsum, out_beg, slices, chunks, recs, rel, sbase, tmp_out;
+    std::vector<long long> bin_start_h;  // first record of each bin (nbins + 1)
+    std::vector<WiSlice> slices_h;
+    std::vector<WiChunk> chunks_h;  // host copies stay alive until their copies run
+    WiQueue queue_h;
+    int grid = 0, nbins = 0;
+    int64_t out_total = 0;  // out-entries of all rows
+    bool counted = false;   // the count pass ran on the resident graph
+};
+
+// The short-side walk of one 2D tile (i,k) (walk_index.hip): out-walk rows
+// (class i, ids of class-k rows) and in-walk rows (class k, ids of class-i
+// rows), each a CSR.
+struct TileWalk {
+    DevBuf out_beg, out_ids, in_beg, in_ids;
+    int64_t rows = 0, cols = 0, x_out = 0, x_in = 0;
+};
+
 struct Ctx {
     int device = 0;
     bool ready = false;
@@ -106,6 +147,8 @@ struct Ctx {
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // around the per-graph index work a count does before its plan
+    hipEvent_t ev_idx[2] = {nullptr, nullptr};
     int num_cus = 0;
     size_t lds_max = 0;
     DevCsr g;
@@ -116,8 +159,10 @@ struct Ctx {
     // per-graph index made the first time a count needs it
     DevBuf side_cnt, side_bsum, side_beg, side_ent, side_row, side_stats;
     bool side_stats_ready = false, side_built = false;
+    bool side_rows_built = false;  // side_row: each walk entry's row (the learned features)
+    WalkIndexState wi;
     uint64_t side_w_ref = 0, side_w_short = 0;
-    int64_t side_entries = 0;  // entries of the walk rows
+    int64_t side_entries = -1;  // entries of the walk rows (-1: not read back yet)
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     DevBuf walk_acc;  // per-vertex sum and max of the short-side walk's list lengths
@@ -181,13 +226,34 @@ struct ResidentRun {
     bool side = false;           // short-side walk
     bool segmented = false;      // an int64 batch counted as int32 segments
     bool starts = false;         // the reference walk reads the start index
-    double index_seconds = 0;    // one-time index work done while preparing
+    bool index_ev = false;       // this count enqueued index work between ctx->ev_idx[0..1]
+    double index_seconds = 0;    // device time of that work (after the count's sync)
 };
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R);
 // Builds the reference walk's start index unless it exists.
-int ensure_start_index(Ctx *ctx, const ResidentRun &R);
+int ensure_start_index(Ctx *ctx, ResidentRun &R);
+// Entries of the short-side walk rows (read back once after a build).
+int side_entry_count(Ctx *ctx, int64_t *n);
 // Builds the short-side walk index of the resident graph if it has none.
 int ensure_walk_index(Ctx *ctx);
+// Writes each walk entry's row (side_row) unless written for this index.
+int ensure_walk_rows(Ctx *ctx);
+void release_walk_scratch(Ctx *ctx);
+// Binned walk-index build (walk_index.hip).  walk_index_binned: the graph
+// has few enough target bins; walk_index_count: the count pass, which also
+// yields both walks' probe totals (synchronises); walk_index_build: the
+// rows from the counted graph (launches only).
+bool walk_index_binned(const Ctx *ctx);
+int walk_index_count(Ctx *ctx);
+int walk_index_build(Ctx *ctx);
+// Builds the walk of a tile from its device CSR and the global out-degrees
+// of its rows' (degR) and columns' (degC) vertices; synchronises once.
+template <typename OffT>
+int tile_walk_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long rows, long long cols,
+                    const int32_t *degR, const int32_t *degC, TileWalk &tw);
+// Whether a tile walk with `cols` column vertices fits the binned build.
+int tile_walk_bins_ok(long long cols);
+
 // Releases the captured batch graph (after a new load or at teardown).
 void drop_captured(Ctx *ctx);
 // Cuts the loaded batch into segments of at most max_edges edges (0: none).
@@ -200,9 +266,10 @@ int read_ctl(Ctx *ctx, const ResidentRun &R, TcCtl &h);
 // Enqueues the fills, plan and counting launches on ctx->stream; `timed`
 // records ctx->ev[0..2] around plan and count (not under capture).
 int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRun &R, bool timed);
-// Fills *r from the control blocks copied back after the count.
-void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_plan,
-                     double t_count, trust_result *r);
+// Fills *r from the control blocks copied back after the count (the stream
+// has been synchronised): seconds = index work of this call + plan + count.
+int finish_resident(Ctx *ctx, ResidentRun &R, const TcCtl &h, double t_plan, double t_count,
+                    trust_result *r);
 
 // Learned scheduler (learned.hip): enqueues features, MLP inference, tier
 // choice and the longest-first counting sort that fill ctx->list and

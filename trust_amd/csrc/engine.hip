@@ -219,7 +219,8 @@ static void fill_args(TcArgs<OffT> &a, Ctx *ctx, const CsrViews &views, const tr
     a.chunk_wave = L.chunk_wave;
     a.rank = p.rank;
     a.nranks = p.total_process;
-    a.force_linear = (p.flags & TRUST_FLAG_LINEAR_PROBING) ? 1 : 0;
+    a.force_linear = (p.flags & TRUST_FLAG_LINEAR_PROBING) ? 1
+                     : (p.flags & TRUST_FLAG_REBUILD_LINEAR) ? 2 : 0;
     a.ctl = ctl_dev;
     a.instr_cycles = instr_cycles;
     a.instr_probes = instr_probes;
@@ -350,7 +351,8 @@ int check_params(const trust_params &p)
     const uint32_t known = TRUST_FLAG_FORCE_SPILL | TRUST_FLAG_OFFSETS64 | TRUST_FLAG_NO_VALIDATE |
                            TRUST_FLAG_WAVE_ONLY | TRUST_FLAG_LINEAR_PROBING | TRUST_FLAG_HIP_GRAPH |
                            TRUST_FLAG_REFERENCE_WALK | TRUST_FLAG_SHORT_WALK |
-                           TRUST_FLAG_OFFSET_LOOKUP;
+                           TRUST_FLAG_OFFSET_LOOKUP | TRUST_FLAG_FRESH_INDEX |
+                           TRUST_FLAG_REBUILD_LINEAR;
     if (p.flags & ~known)
         return fail(TRUST_ERR_ARG, "unknown flag bits 0x%x", p.flags & ~known);
     if ((p.flags & TRUST_FLAG_REFERENCE_WALK) && (p.flags & TRUST_FLAG_SHORT_WALK))
@@ -380,81 +382,154 @@ double alg_bytes_short(int64_t V, int64_t E, uint64_t W, bool off64, int64_t X)
            4.0 * (double)W;
 }
 
+// Marks the start of the per-graph index work of this count (once).
+static int index_begin(Ctx *ctx, ResidentRun &R)
+{
+    if (!R.index_ev) {
+        HIPCHK(hipEventRecord(ctx->ev_idx[0], ctx->stream));
+        R.index_ev = true;
+    }
+    return TRUST_OK;
+}
+
 // The walk of an unpartitioned count: the short side when forced or when
 // it makes at most kShortWalkShare of the reference walk's probes.  The
-// probe totals and the in-walk rows depend on the graph alone, so they are
-// made once per loaded graph, here, before any timed region, and the time
-// is reported as index_seconds.
+// probe totals and the in-walk rows depend on the graph alone, so a context
+// makes them once per loaded graph (or again under TRUST_FLAG_FRESH_INDEX);
+// the count that makes them includes their device time in its seconds.
 static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
 {
     DevCsr &g = ctx->g;
-    const auto t0 = std::chrono::steady_clock::now();
-    bool worked = false;
+    const bool binned = walk_index_binned(ctx);
+    int rc;
     if (!force && !ctx->side_stats_ready) {
-        worked = true;
-        int rc = ctx->side_stats.ensure(4 * sizeof(unsigned long long));
-        if (rc)
+        if ((rc = index_begin(ctx, R)))
             return rc;
-        unsigned long long *st = (unsigned long long *)ctx->side_stats.p;
-        if (g.off64)
-            HIPCHK(launch_side_stats<int64_t>((const int64_t *)g.beg.p, (const int32_t *)g.adj.p,
-                                              g.V, st, ctx->stream));
-        else
-            HIPCHK(launch_side_stats<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
-                                              g.V, st, ctx->stream));
-        unsigned long long h[2];
-        HIPCHK(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        ctx->side_w_ref = h[0];
-        ctx->side_w_short = h[1];
-        ctx->side_stats_ready = true;
+        if (binned) {
+            // the binned build's count pass yields the probe totals too
+            if ((rc = walk_index_count(ctx)))
+                return rc;
+        } else {
+            rc = ctx->side_stats.ensure(4 * sizeof(unsigned long long));
+            if (rc)
+                return rc;
+            unsigned long long *st = (unsigned long long *)ctx->side_stats.p;
+            if (g.off64)
+                HIPCHK(launch_side_stats<int64_t>((const int64_t *)g.beg.p,
+                                                  (const int32_t *)g.adj.p, g.V, st, ctx->stream));
+            else
+                HIPCHK(launch_side_stats<int32_t>((const int32_t *)g.beg.p,
+                                                  (const int32_t *)g.adj.p, g.V, st, ctx->stream));
+            unsigned long long h[2];
+            HIPCHK(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->side_w_ref = h[0];
+            ctx->side_w_short = h[1];
+            ctx->side_stats_ready = true;
+        }
     }
     R.side = force || (double)ctx->side_w_short <= kShortWalkShare * (double)ctx->side_w_ref;
     if (R.side && !ctx->side_built) {
-        worked = true;
-        const size_t ob = g.off64 ? sizeof(int64_t) : sizeof(int32_t);
-        int rc = ctx->side_cnt.ensure(2 * sizeof(int) * (size_t)g.V);
-        if (!rc)
-            rc = ctx->side_bsum.ensure(sizeof(long long) * (size_t)(side_scan_blocks(g.V) + 1));
-        if (!rc)
-            rc = ctx->side_beg.ensure(ob * (size_t)(g.V + 1));
-        if (!rc)
-            rc = ctx->side_ent.ensure(2 * ob * (size_t)g.E);  // WalkEntry: start, length
-        if (!rc)
-            rc = ctx->side_row.ensure(sizeof(int32_t) * (size_t)g.E);
-        if (rc)
+        if ((rc = index_begin(ctx, R)))
             return rc;
-        if (g.off64)
-            HIPCHK(launch_side_build<int64_t>((const int64_t *)g.beg.p, (const int32_t *)g.adj.p,
-                                              g.V, (int *)ctx->side_cnt.p,
-                                              (long long *)ctx->side_bsum.p,
-                                              (int64_t *)ctx->side_beg.p,
-                                              (WalkEntry<int64_t> *)ctx->side_ent.p,
-                                              (int32_t *)ctx->side_row.p, ctx->stream));
-        else
-            HIPCHK(launch_side_build<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p,
-                                              g.V, (int *)ctx->side_cnt.p,
-                                              (long long *)ctx->side_bsum.p,
-                                              (int32_t *)ctx->side_beg.p,
-                                              (WalkEntry<int32_t> *)ctx->side_ent.p,
-                                              (int32_t *)ctx->side_row.p, ctx->stream));
-        if (g.off64) {
-            int64_t n = 0;
-            HIPCHK(hipMemcpyAsync(&n, (const int64_t *)ctx->side_beg.p + g.V, sizeof(n),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            ctx->side_entries = n;
-        } else {
-            int32_t n = 0;
-            HIPCHK(hipMemcpyAsync(&n, (const int32_t *)ctx->side_beg.p + g.V, sizeof(n),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            ctx->side_entries = n;
+        const size_t ob = g.off64 ? sizeof(int64_t) : sizeof(int32_t);
+        bool done = false;
+        if (binned) {
+            if (!ctx->wi.counted && (rc = walk_index_count(ctx)))
+                return rc;
+            rc = walk_index_build(ctx);
+            if (rc == TRUST_ERR_OOM) {  // not enough HBM for its scratch: the atomic build
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+                release_walk_scratch(ctx);
+            } else if (rc) {
+                return rc;
+            } else {
+                done = true;
+            }
+        }
+        if (!done) {
+            rc = ctx->side_cnt.ensure(2 * sizeof(int) * (size_t)g.V);
+            if (!rc)
+                rc = ctx->side_bsum.ensure(sizeof(long long) * (size_t)(scan_blocks(g.V) + 1));
+            if (!rc)
+                rc = ctx->side_beg.ensure(ob * (size_t)(g.V + 1));
+            if (!rc)
+                rc = ctx->side_ent.ensure(2 * ob * (size_t)g.E);  // WalkEntry: start, length
+            if (rc)
+                return rc;
+            if (g.off64)
+                HIPCHK(launch_side_build<int64_t>((const int64_t *)g.beg.p,
+                                                  (const int32_t *)g.adj.p, g.V,
+                                                  (int *)ctx->side_cnt.p,
+                                                  (long long *)ctx->side_bsum.p,
+                                                  (int64_t *)ctx->side_beg.p,
+                                                  (WalkEntry<int64_t> *)ctx->side_ent.p,
+                                                  ctx->stream));
+            else
+                HIPCHK(launch_side_build<int32_t>((const int32_t *)g.beg.p,
+                                                  (const int32_t *)g.adj.p, g.V,
+                                                  (int *)ctx->side_cnt.p,
+                                                  (long long *)ctx->side_bsum.p,
+                                                  (int32_t *)ctx->side_beg.p,
+                                                  (WalkEntry<int32_t> *)ctx->side_ent.p,
+                                                  ctx->stream));
+            ctx->side_entries = -1;  // read back when needed, outside the timed work
         }
         ctx->side_built = true;
+        ctx->side_rows_built = false;
     }
-    R.index_seconds =
-        worked ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() : 0.0;
+    return TRUST_OK;
+}
+
+// Frees the binned build's scratch (records, ranks, bases, out-part copy).
+void release_walk_scratch(Ctx *ctx)
+{
+    WalkIndexState &w = ctx->wi;
+    w.recs.release();
+    w.rel.release();
+    w.sbase.release();
+    w.tmp_out.release();
+}
+
+int ensure_walk_rows(Ctx *ctx)
+{
+    const DevCsr &g = ctx->g;
+    if (!ctx->side_built)
+        return fail(TRUST_ERR_ARG, "short-side walk rows not built");
+    if (ctx->side_rows_built || g.V <= 0)
+        return TRUST_OK;
+    int rc = ctx->side_row.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(g.E, 1));
+    if (rc)
+        return rc;
+    if (g.off64)
+        HIPCHK(launch_side_rows<int64_t>((const int64_t *)ctx->side_beg.p, g.V,
+                                         (int32_t *)ctx->side_row.p, ctx->stream));
+    else
+        HIPCHK(launch_side_rows<int32_t>((const int32_t *)ctx->side_beg.p, g.V,
+                                         (int32_t *)ctx->side_row.p, ctx->stream));
+    ctx->side_rows_built = true;
+    return TRUST_OK;
+}
+
+int side_entry_count(Ctx *ctx, int64_t *n)
+{
+    const DevCsr &g = ctx->g;
+    if (ctx->side_entries < 0) {
+        if (g.off64) {
+            int64_t x = 0;
+            HIPCHK(hipMemcpyAsync(&x, (const int64_t *)ctx->side_beg.p + g.V, sizeof(x),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->side_entries = x;
+        } else {
+            int32_t x = 0;
+            HIPCHK(hipMemcpyAsync(&x, (const int32_t *)ctx->side_beg.p + g.V, sizeof(x),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->side_entries = x;
+        }
+    }
+    *n = ctx->side_entries;
     return TRUST_OK;
 }
 
@@ -463,7 +538,21 @@ int ensure_walk_index(Ctx *ctx)
     if (ctx->g.V <= 0 || ctx->g.E <= 0 || ctx->side_built)
         return TRUST_OK;
     ResidentRun R;
-    return prepare_side(ctx, true, R);
+    int rc = prepare_side(ctx, true, R);
+    if (!rc)
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    return rc;
+}
+
+// Forgets the per-graph indexes of the resident graph, so that the next
+// count makes them again (TRUST_FLAG_FRESH_INDEX: a count of a freshly
+// loaded graph without reloading it).
+static void drop_indexes(Ctx *ctx)
+{
+    ctx->side_stats_ready = ctx->side_built = ctx->side_rows_built = false;
+    ctx->wi.counted = false;
+    ctx->side_entries = -1;
+    ctx->ref_built = false;
 }
 
 int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
@@ -479,7 +568,13 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
                     (long long)g.max_deg);
     if (mode == kModeBatch && (ctx->batch_ngraphs < 1 || !ctx->batch_vbase.p))
         return fail(TRUST_ERR_ARG, "context holds no batch");
+    if (p.flags & TRUST_FLAG_FRESH_INDEX) {
+        // the previous count may still read the indexes being rebuilt
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        drop_indexes(ctx);
+    }
     R.side = false;
+    R.index_ev = false;
     R.index_seconds = 0;
     if (!(p.flags & TRUST_FLAG_REFERENCE_WALK) && g.V > 0 && g.E > 0) {
         rc = prepare_side(ctx, (p.flags & TRUST_FLAG_SHORT_WALK) != 0, R);
@@ -516,19 +611,21 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
     // the lookup walk reuses the table's first 64 ids as walk ids for free.
     R.starts = !R.side && !(p.flags & TRUST_FLAG_OFFSET_LOOKUP) && g.V > 0 && g.E > 0 &&
                !g.off64 && ctx->batch_ngraphs == 0;
-    if (R.starts) {
-        const auto t0 = std::chrono::steady_clock::now();
-        const bool built0 = ctx->ref_built;
-        if ((rc = ensure_start_index(ctx, R)))
+    if (R.starts && (rc = ensure_start_index(ctx, R)))
+        return rc;
+    // the learned plan's features read each walk entry's row, and size their
+    // pass by the walk rows' entries
+    if (R.side && R.learned) {
+        int64_t n;
+        if ((rc = side_entry_count(ctx, &n)) || (rc = ensure_walk_rows(ctx)))
             return rc;
-        if (!built0)
-            R.index_seconds +=
-                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    if (R.index_ev)
+        HIPCHK(hipEventRecord(ctx->ev_idx[1], ctx->stream));
     return TRUST_OK;
 }
 
-int ensure_start_index(Ctx *ctx, const ResidentRun &R)
+int ensure_start_index(Ctx *ctx, ResidentRun &R)
 {
     const DevCsr &g = ctx->g;
     if (ctx->ref_built)
@@ -538,9 +635,10 @@ int ensure_start_index(Ctx *ctx, const ResidentRun &R)
     int rc = ctx->ref_ent.ensure(sizeof(WalkEntry<int32_t>) * (size_t)g.E);
     if (rc)
         return rc;
+    if ((rc = index_begin(ctx, R)))
+        return rc;
     HIPCHK(launch_start_index<int32_t>((const int32_t *)g.beg.p, (const int32_t *)g.adj.p, g.E,
                                        (WalkEntry<int32_t> *)ctx->ref_ent.p, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->ref_built = true;
     return TRUST_OK;
 }
@@ -593,13 +691,24 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
     return rc;
 }
 
-void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_plan,
-                     double t_count, trust_result *r)
+int finish_resident(Ctx *ctx, ResidentRun &R, const TcCtl &h, double t_plan, double t_count,
+                    trust_result *r)
 {
+    if (R.index_ev) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_idx[0], ctx->ev_idx[1]));
+        R.index_seconds = ms * 1e-3;
+    }
     if (!r)
-        return;
+        return TRUST_OK;
     const DevCsr &g = ctx->g;
     const LaunchPlan &L = R.L;
+    int64_t X = 0;
+    if (R.side) {
+        int rc = side_entry_count(ctx, &X);
+        if (rc)
+            return rc;
+    }
     memset(r, 0, sizeof(*r));
     r->vertices = g.V;
     r->edges = g.E;
@@ -607,10 +716,11 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
     r->probes = h.probes;
     r->plan_seconds = t_plan;
     r->count_seconds = t_count;
-    r->seconds = t_plan + t_count;
+    r->index_seconds = R.index_seconds;
+    r->seconds = R.index_seconds + t_plan + t_count;
     r->h2d_seconds = ctx->h2d_seconds;
     r->gteps = r->seconds > 0 ? (double)g.E / r->seconds / 1e9 : 0.0;
-    r->alg_bytes = R.side ? alg_bytes_short(g.V, g.E, h.probes, g.off64, ctx->side_entries)
+    r->alg_bytes = R.side ? alg_bytes_short(g.V, g.E, h.probes, g.off64, X)
                  : R.starts ? alg_bytes_starts(g.V, g.E, h.probes, g.off64 && !R.segmented)
                           : alg_bytes(g.V, g.E, h.probes, g.off64 && !R.segmented);
     r->n_spill = h.list_begin[kTierSpill + 1] - h.list_begin[kTierSpill];
@@ -622,7 +732,8 @@ void finish_resident(Ctx *ctx, const ResidentRun &R, const TcCtl &h, double t_pl
     r->wave_slots = L.main.wave_slots;
     r->schedule_used = R.learned ? TRUST_SCHED_LEARNED : TRUST_SCHED_DEGREE;
     r->walk_used = R.side ? 1 : 0;
-    r->index_seconds = R.index_seconds;
+    r->linear_rebuilds = h.rebuilds;
+    return TRUST_OK;
 }
 
 // Count over the context's resident graph.
@@ -654,8 +765,7 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
     float t_plan = 0, t_count = 0;
     HIPCHK(hipEventElapsedTime(&t_plan, ctx->ev[0], ctx->ev[1]));
     HIPCHK(hipEventElapsedTime(&t_count, ctx->ev[1], ctx->ev[2]));
-    finish_resident(ctx, R, h, t_plan * 1e-3, t_count * 1e-3, r);
-    return TRUST_OK;
+    return finish_resident(ctx, R, h, t_plan * 1e-3, t_count * 1e-3, r);
 }
 
 int ctx_init(Ctx *ctx, int device)
@@ -680,6 +790,8 @@ int ctx_init(Ctx *ctx, int device)
     HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     for (auto &ev : ctx->ev)
         HIPCHK(hipEventCreate(&ev));
+    for (auto &ev : ctx->ev_idx)
+        HIPCHK(hipEventCreate(&ev));
     ctx->ready = true;
     return TRUST_OK;
 }
@@ -696,6 +808,8 @@ Ctx::~Ctx()
     g.beg.release();
     g.adj.release();
     for (auto &ev : this->ev)
+        (void)hipEventDestroy(ev);
+    for (auto &ev : this->ev_idx)
         (void)hipEventDestroy(ev);
     (void)hipEventDestroy(ev_fork);
     (void)hipEventDestroy(ev_join);
@@ -718,9 +832,8 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
     drop_captured(ctx);
     ctx->batch_ngraphs = 0;
     ctx->segs.clear();
-    ctx->ref_built = false;  // the start index belongs to the graph
-    ctx->judge = -1;  // the judgement belongs to the graph; the model stays
-    ctx->side_stats_ready = ctx->side_built = false;  // the walk index belongs to the graph
+    drop_indexes(ctx);  // the walk statistics and indexes belong to the graph
+    ctx->judge = -1;    // the judgement belongs to the graph; the model stays
     auto t0 = std::chrono::steady_clock::now();
     int rc = upload_csr(ctx, ctx->g, beg, adj, V, E, (flags & TRUST_FLAG_OFFSETS64) != 0,
                         ctx->stream);

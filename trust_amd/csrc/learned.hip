@@ -29,6 +29,7 @@
 
 #include "engine.h"
 #include "mlp.h"
+#include "wave_ops.h"
 
 namespace trust {
 
@@ -36,14 +37,6 @@ constexpr int kBins = 64;
 constexpr int kKeys = kNumTiers * kBins;  // 256
 constexpr unsigned short kKeyNone = 0xFFFF;
 
-__device__ __forceinline__ int lane64() { return (int)__lane_id(); }
-
-__device__ __forceinline__ unsigned long long wsum_u64(unsigned long long x)
-{
-    for (int off = 32; off > 0; off >>= 1)
-        x += __shfl_xor(x, off);
-    return x;
-}
 __device__ __forceinline__ int wmax_i(int x)
 {
     for (int off = 32; off > 0; off >>= 1)
@@ -59,24 +52,6 @@ __device__ __forceinline__ int wmax_i(int x)
 // lane adds d+(v) of its position into the owner's LDS slot.
 constexpr int kFeatWin = 4;
 
-__device__ __forceinline__ int fw_scan_max(int x)  // inclusive prefix max, x >= 0
-{
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
-    return x;
-}
-
-__device__ __forceinline__ void fw_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <typename OffT>
 __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ beg,
                                                        const int32_t *__restrict__ adj,
@@ -86,7 +61,7 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
     __shared__ int mark_s[4][64 * kFeatWin];
     __shared__ unsigned long long sum_s[4][64];
     __shared__ int max_s[4][64];
-    const int wave = threadIdx.x / 64, lane = lane64();
+    const int wave = threadIdx.x / 64, lane = lane_id();
     int *mark = mark_s[wave];
     unsigned long long *lsum = sum_s[wave];
     int *lmax = max_s[wave];
@@ -106,16 +81,16 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
 #pragma unroll
             for (int k = 0; k < kFeatWin; k++)
                 mark[k * 64 + lane] = 0;
-            fw_sync();
+            wave_sync();
             // list starts inside this span (an empty list starts where the
             // next one does, so only non-empty lists mark)
             if (d > 0 && (long long)s >= p && (long long)s < p + 64 * kFeatWin)
                 mark[(int)((long long)s - p)] = lane;
-            fw_sync();
+            wave_sync();
             int own[kFeatWin], v[kFeatWin];
 #pragma unroll
             for (int k = 0; k < kFeatWin; k++) {
-                own[k] = max(fw_scan_max(mark[k * 64 + lane]), carry);
+                own[k] = max(wave_scan_max(mark[k * 64 + lane]), carry);
                 carry = __shfl(own[k], 63);
                 const long long pos = p + k * 64 + lane;
                 v[k] = pos < e1 ? adj[pos] : -1;
@@ -131,9 +106,9 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
                     atomicMax(&lmax[own[k]], dv[k]);
                 }
             }
-            fw_sync();
+            wave_sync();
         }
-        fw_sync();
+        wave_sync();
         if (lane < n) {
             const long long u = u0 + lane;
             const float dd = (float)(begD[u + 1] - begD[u]);  // table size d+(u)
@@ -141,7 +116,7 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
             feat[3 * u + 1] = log2f(1.f + (float)lsum[lane]) * kFeatScale;
             feat[3 * u + 2] = log2f(1.f + (float)lmax[lane]) * kFeatScale;
         }
-        fw_sync();
+        wave_sync();
     }
 }
 
@@ -157,7 +132,7 @@ __global__ void __launch_bounds__(256) walk_feat_kernel(const int32_t *__restric
                                                         unsigned long long *__restrict__ wsum,
                                                         unsigned int *__restrict__ wmax)
 {
-    const int lane = lane64();
+    const int lane = lane_id();
     const long long nw = (long long)gridDim.x * (blockDim.x / 64);
     for (long long w = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w * 64 < X;
          w += nw) {
@@ -184,7 +159,7 @@ __global__ void __launch_bounds__(256) walk_feat_kernel(const int32_t *__restric
             if (lane >= off && y > key)
                 key = y;
         }
-        const int hl = fw_scan_max(head ? lane : 0);  // this lane's segment head
+        const int hl = wave_scan_max(head ? lane : 0);  // this lane's segment head
         const int before = __shfl(inc, hl > 0 ? hl - 1 : 0);
         const int seg = hl > 0 ? inc - before : inc;
         if (last) {
@@ -255,7 +230,7 @@ __global__ void __launch_bounds__(256) cost_kernel(const OffT *__restrict__ beg,
     }
     for (int off = 32; off > 0; off >>= 1)
         local += __shfl_xor(local, off);
-    if (lane64() == 0)
+    if (lane_id() == 0)
         part[threadIdx.x / 64] = local;
     __syncthreads();
     if (threadIdx.x == 0)
@@ -410,7 +385,7 @@ __global__ void __launch_bounds__(256) key_scatter_kernel(const unsigned short *
 {
     __shared__ long long run[kKeys];
     __shared__ unsigned short wcnt[4][kKeys];
-    const int wave = threadIdx.x / 64, lane = lane64();
+    const int wave = threadIdx.x / 64, lane = lane_id();
     for (int i = threadIdx.x; i < kKeys; i += blockDim.x) {
         run[i] = offsets[(long long)i * nb + blockIdx.x];
         wcnt[0][i] = wcnt[1][i] = wcnt[2][i] = wcnt[3][i] = 0;
@@ -473,7 +448,9 @@ static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s, bool walk)
     unsigned long long *wsum = (unsigned long long *)ctx->walk_acc.p;
     unsigned int *wmax = (unsigned int *)(wsum + V);
     HIPCHK(launch_fill_u32((uint32_t *)ctx->walk_acc.p, 0u, 3 * V, s));
-    const long long X = ctx->side_entries;
+    int64_t X = 0;
+    if ((rc = side_entry_count(ctx, &X)))  // read back already when under capture
+        return rc;
     if (X > 0) {
         // the length field of every WalkEntry<OffT> of the walk index
         const int32_t *deg = (const int32_t *)((const char *)ctx->side_ent.p +
@@ -491,8 +468,11 @@ static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s, bool walk)
 
 static int features_dispatch(Ctx *ctx, float *feat, hipStream_t s, bool walk = false)
 {
-    if (walk && !ctx->side_built)
-        return fail(TRUST_ERR_ARG, "short-side walk rows not built");
+    if (walk) {
+        int rc = ensure_walk_rows(ctx);  // written before any capture (prepare_resident)
+        if (rc)
+            return rc;
+    }
     return ctx->g.off64 ? enqueue_features<int64_t>(ctx, feat, s, walk)
                         : enqueue_features<int32_t>(ctx, feat, s, walk);
 }

@@ -138,16 +138,52 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
             }
         }
     } guard{slots};
+    // The short-side walk by global out-degree (default; REFERENCE_WALK keeps
+    // the reference's): job (i,j,k) is two counts, the out-walk of tile (i,k)
+    // with tables of tile (i,j) probing tile (k,j), and its in-walk with
+    // tables of tile (k,j) probing tile (i,j).  Both read only tiles (i,j)
+    // and (k,j); the walk tile is needed only to build its walk, once per i.
+    bool short_walk = !(p.flags & TRUST_FLAG_REFERENCE_WALK);
+    std::vector<int64_t> rows_of((size_t)n), deg_off((size_t)n + 1, 0);
+    for (int c = 0; c < n; c++) {
+        rows_of[c] = tiles[(size_t)c * n].vertices;
+        for (int b = 1; b < n; b++)
+            if (tiles[(size_t)c * n + b].vertices != rows_of[c])
+                short_walk = false;  // tiles of one class disagree on its rows: no global degrees
+        short_walk = short_walk && tile_walk_bins_ok(rows_of[c]);
+        deg_off[c + 1] = deg_off[c] + rows_of[c];
+    }
+    DevBuf degs;
+    std::vector<TileWalk> walks(short_walk ? (size_t)n : 0);
+    std::vector<int> walk_row(walks.size(), -1);  // the tile row i each walk was built for
+    if (short_walk) {
+        // global out-degree of every vertex, class by class (sum over its tile row)
+        std::vector<int32_t> d((size_t)deg_off[n]);
+        for (int c = 0; c < n; c++) {
+            int32_t *dc = d.data() + deg_off[c];
+#pragma omp parallel for schedule(static)
+            for (int64_t r = 0; r < rows_of[c]; r++) {
+                int64_t x = 0;
+                for (int b = 0; b < n; b++)
+                    x += tiles[(size_t)c * n + b].beg[r + 1] - tiles[(size_t)c * n + b].beg[r];
+                dc[r] = (int32_t)std::min<int64_t>(x, INT32_MAX);
+            }
+        }
+        if ((rc = degs.ensure(sizeof(int32_t) * std::max<size_t>(d.size(), 1))))
+            return rc;
+        HIPCHK(hipMemcpy(degs.p, d.data(), sizeof(int32_t) * d.size(), hipMemcpyHostToDevice));
+    }
     DevBuf ctls;
-    rc = ctls.ensure(sizeof(TcCtl) * (size_t)njobs);
+    const long long nctl = short_walk ? 2 * njobs : njobs;
+    rc = ctls.ensure(sizeof(TcCtl) * (size_t)nctl);
     if (rc)
         return rc;
 
-    auto job_tiles = [n](long long job, int out[3]) {
+    auto job_tiles = [n, short_walk](long long job, int out[3]) {
         int i = (int)(job / ((long long)n * n)), j = (int)(job / n % n), k = (int)(job % n);
-        out[0] = i * n + j;  // hash
-        out[1] = i * n + k;  // walk
-        out[2] = k * n + j;  // probe
+        out[0] = i * n + j;  // hash (the in-walk's probe)
+        out[1] = short_walk ? -1 : i * n + k;  // walk (the short walk reads its own index)
+        out[2] = k * n + j;  // probe (the in-walk's hash)
     };
     // Make tile t resident for job `job`; never evicts tiles of jobs >= pin_from.
     auto acquire = [&](int t, long long job, const int keep[6], int &slot_out) -> int {
@@ -184,36 +220,80 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         return TRUST_OK;
     };
 
+    auto count_view = [&](const CsrViews &v, int64_t rows, TcCtl *ctl) -> int {
+        return off64 ? enqueue_count<int64_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
+                                              nullptr, false, ctx->stream, nullptr, nullptr)
+                     : enqueue_count<int32_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
+                                              nullptr, false, ctx->stream, nullptr, nullptr);
+    };
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     for (long long job = 0; job < njobs; job++) {
+        const int ji = (int)(job / ((long long)n * n)), jj = (int)(job / n % n), jk = (int)(job % n);
         int cur[3], nxt[3] = {-1, -1, -1};
         job_tiles(job, cur);
         if (job + 1 < njobs)
             job_tiles(job + 1, nxt);
+        if (short_walk && walk_row[(size_t)jk] != ji) {
+            // the walk of tile (i,k), built once for the jobs (i, *, k)
+            const int wt = ji * n + jk;
+            const int keepw[6] = {wt, cur[0], cur[2], -1, -1, -1};
+            int ws;
+            if ((rc = acquire(wt, job, keepw, ws)))
+                return rc;
+            slots[ws].last_job = job;
+            HIPCHK(hipStreamWaitEvent(ctx->stream, slots[ws].ready, 0));
+            const DevCsr &wg = slots[ws].g;
+            const int32_t *dg = (const int32_t *)degs.p;
+            rc = off64 ? tile_walk_build<int64_t>(ctx, (const int64_t *)wg.beg.p,
+                                                  (const int32_t *)wg.adj.p, rows_of[ji],
+                                                  rows_of[jk], dg + deg_off[ji], dg + deg_off[jk],
+                                                  walks[(size_t)jk])
+                       : tile_walk_build<int32_t>(ctx, (const int32_t *)wg.beg.p,
+                                                  (const int32_t *)wg.adj.p, rows_of[ji],
+                                                  rows_of[jk], dg + deg_off[ji], dg + deg_off[jk],
+                                                  walks[(size_t)jk]);
+            if (rc)
+                return rc;
+            HIPCHK(hipEventRecord(slots[ws].released, ctx->stream));
+            walk_row[(size_t)jk] = ji;
+        }
         const int keep[6] = {cur[0], cur[1], cur[2], -1, -1, -1};
-        int sl[3];
+        int sl[3] = {-1, -1, -1};
         for (int q = 0; q < 3; q++) {
+            if (cur[q] < 0)
+                continue;
             rc = acquire(cur[q], job, keep, sl[q]);
             if (rc)
                 return rc;
         }
         for (int q = 0; q < 3; q++) {
+            if (sl[q] < 0)
+                continue;
             slots[sl[q]].last_job = job;
             HIPCHK(hipStreamWaitEvent(ctx->stream, slots[sl[q]].ready, 0));
         }
-        CsrViews v{slots[sl[0]].g.beg.p, (const int32_t *)slots[sl[0]].g.adj.p,
-                   slots[sl[1]].g.beg.p, (const int32_t *)slots[sl[1]].g.adj.p,
-                   slots[sl[2]].g.beg.p, (const int32_t *)slots[sl[2]].g.adj.p};
-        const int64_t rows = tiles[cur[0]].vertices;
-        TcCtl *ctl = (TcCtl *)ctls.p + job;
-        rc = off64 ? enqueue_count<int64_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
-                                            nullptr, false, ctx->stream, nullptr, nullptr)
-                   : enqueue_count<int32_t>(ctx, v, rows, false, p, L, ctl, kModeCount, nullptr,
-                                            nullptr, false, ctx->stream, nullptr, nullptr);
-        if (rc)
-            return rc;
+        const DevCsr &th = slots[sl[0]].g, &tp = slots[sl[2]].g;
+        if (short_walk) {
+            const TileWalk &w = walks[(size_t)jk];
+            CsrViews vo{th.beg.p, (const int32_t *)th.adj.p, w.out_beg.p,
+                        (const int32_t *)w.out_ids.p, tp.beg.p, (const int32_t *)tp.adj.p};
+            if ((rc = count_view(vo, rows_of[ji], (TcCtl *)ctls.p + 2 * job)))
+                return rc;
+            CsrViews vi{tp.beg.p, (const int32_t *)tp.adj.p, w.in_beg.p,
+                        (const int32_t *)w.in_ids.p, th.beg.p, (const int32_t *)th.adj.p};
+            if ((rc = count_view(vi, rows_of[jk], (TcCtl *)ctls.p + 2 * job + 1)))
+                return rc;
+        } else {
+            const DevCsr &tw = slots[sl[1]].g;
+            CsrViews v{th.beg.p, (const int32_t *)th.adj.p, tw.beg.p, (const int32_t *)tw.adj.p,
+                       tp.beg.p, (const int32_t *)tp.adj.p};
+            if ((rc = count_view(v, tiles[cur[0]].vertices, (TcCtl *)ctls.p + job)))
+                return rc;
+        }
+        (void)jj;
         for (int q = 0; q < 3; q++)
-            HIPCHK(hipEventRecord(slots[sl[q]].released, ctx->stream));
+            if (sl[q] >= 0)
+                HIPCHK(hipEventRecord(slots[sl[q]].released, ctx->stream));
         // Prefetch the next job's tiles while this one counts.
         if (job + 1 < njobs) {
             const int keep2[6] = {cur[0], cur[1], cur[2], nxt[0], nxt[1], nxt[2]};
@@ -222,6 +302,8 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                 bool fits = cache_slots >= 6;
                 if (!fits)
                     break;  // too few slots to prefetch without evicting the running job
+                if (nxt[q] < 0)
+                    continue;
                 rc = acquire(nxt[q], job + 1, keep2, dummy);
                 if (rc)
                     return rc;
@@ -229,18 +311,28 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         }
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    std::vector<TcCtl> h((size_t)njobs);
-    HIPCHK(hipMemcpyAsync(h.data(), ctls.p, sizeof(TcCtl) * (size_t)njobs, hipMemcpyDeviceToHost,
+    std::vector<TcCtl> h((size_t)nctl);
+    HIPCHK(hipMemcpyAsync(h.data(), ctls.p, sizeof(TcCtl) * (size_t)nctl, hipMemcpyDeviceToHost,
                           ctx->stream));
+    if (short_walk)  // fold each job's in-walk count into its out-walk count
+        for (long long j = 0; j < njobs; j++) {
+            TcCtl &a = h[(size_t)(2 * j)];
+            const TcCtl &b = h[(size_t)(2 * j + 1)];
+            a.count += b.count;
+            a.probes += b.probes;
+            a.rebuilds += b.rebuilds;
+        }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    uint64_t total = 0, probes = 0;
+    uint64_t total = 0, probes = 0, rebuilds = 0;
     for (long long j = 0; j < njobs; j++) {
-        total += h[j].count;
-        probes += h[j].probes;
+        const TcCtl &x = h[(size_t)(short_walk ? 2 * j : j)];
+        total += x.count;
+        probes += x.probes;
+        rebuilds += x.rebuilds;
         if (job_counts)
-            job_counts[j] = h[j].count;
+            job_counts[j] = x.count;
     }
     if (r) {
         memset(r, 0, sizeof(*r));
@@ -256,6 +348,8 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         r->blocks = L.main.blocks;
         r->hub_slots = 1 << L.main.group_lg;
         r->wave_slots = L.main.wave_slots;
+        r->linear_rebuilds = rebuilds;
+        r->walk_used = short_walk ? 1 : 0;
     }
     return TRUST_OK;
 }
@@ -272,8 +366,6 @@ extern "C" int trust_count_tiles(const trust_csr *tiles, int n, const trust_para
     int rc = check_params(*p);
     if (rc)
         return rc;
-    if (p->flags & TRUST_FLAG_SHORT_WALK)  // the in-walk rows of a tile job would span tiles
-        return fail(TRUST_ERR_ARG, "the short-side walk needs an unpartitioned graph");
     Ctx ctx;
     rc = ctx_init(&ctx, p->device);
     if (rc)

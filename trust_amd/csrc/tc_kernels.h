@@ -42,7 +42,8 @@ struct TcCtl {
     unsigned long long probes;      // hash lookups (W)
     unsigned int next[8];           // queue cursors, one per tier
     long long list_begin[8];        // tier t occupies list[list_begin[t], list_begin[t+1])
-    unsigned long long pad[4];
+    unsigned long long rebuilds;    // tables rebuilt with linear probing after a cuckoo attempt
+    unsigned long long pad[3];
     unsigned int shard[kQueueShards * kShardStride];  // wave-queue cursors
 };
 
@@ -82,7 +83,8 @@ struct TcArgs {
     int chunk_wave;          // vertices per grab, wave tier
     int rank;
     int nranks;
-    int force_linear;        // 1: linear-probing tables only (no cuckoo)
+    int force_linear;        // 1: linear-probing tables only (no cuckoo); 2: every cuckoo
+                             // table is rebuilt with linear probing (a forced failure)
     // batch of graphs (nullable): graph g owns vertices [vbase[g], vbase[g+1])
     const long long *graph_vbase;
     // graph_page[p] = the graph holding vertex min(p << shift, V - 1)
@@ -135,19 +137,29 @@ hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
                                  hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream_t stream);
 
+// Exclusive scan of a[i] (+ b[i] when b is not null), i < n, into
+// out[0, n], out[n] = the total; bsum holds scan_blocks(n) + 1 long longs
+// (walk_index.hip).  Launches only.
+long long scan_blocks(long long n);
+template <typename OutT>
+hipError_t launch_scan(const int *a, const int *b, long long n, long long *bsum, OutT *out,
+                       hipStream_t stream);
+
 // Builds the walk rows of the short-side walk for an oriented CSR (the walk
-// view of the three-view kernel): row x = the v in N+(x) with
-// d+(v) <= d+(x), then the u with u -> x and d+(u) < d+(x), rows with
-// d+ < 2 contributing nothing.  cnt: 2V ints of scratch; bsum:
-// side_scan_blocks(V) + 1 long longs; begX: V + 1 offsets; entX, rowX: up
-// to E each (the start and length d+ of the entry's list in adj; the row).
-// Launches only (no host sync).
-constexpr int kSideScanPer = 8192;  // vertices per scan workgroup
-inline long long side_scan_blocks(long long V) { return (V + kSideScanPer - 1) / kSideScanPer; }
+// view of the three-view kernel) with one global atomic per in-entry to
+// count its row and one to claim its slot: the build for graphs with too
+// many target bins for the binned build (walk_index.hip).  Row x = the v in
+// N+(x) with d+(v) <= d+(x), then the u with u -> x and d+(u) < d+(x), rows
+// with d+ < 2 contributing nothing.  cnt: 2V ints of scratch; bsum:
+// scan_blocks(V) + 1 long longs; begX: V + 1 offsets; entX: up to E (the
+// start and length d+ of the entry's list in adj).  Launches only.
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begX, WalkEntry<OffT> *entX, int32_t *rowX,
+                             long long *bsum, OffT *begX, WalkEntry<OffT> *entX,
                              hipStream_t stream);
+// rowX[at] = the row x whose entries [begX[x], begX[x+1]) hold position at.
+template <typename OffT>
+hipError_t launch_side_rows(const OffT *begX, long long V, int32_t *rowX, hipStream_t stream);
 // Start index of the reference walk: for e < E, entR[e] = {begP[adj[e]],
 // its list length}.  Launches only.
 template <typename OffT>

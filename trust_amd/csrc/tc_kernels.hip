@@ -37,67 +37,12 @@
 #include <hip/hip_runtime.h>
 
 #include "tc_kernels.h"
+#include "wave_ops.h"
 
 namespace trust {
 
 constexpr int kEmpty = -1;
 constexpr int kNoProbe = -2;  // a window position past the end: matches no slot
-
-// ---------------------------------------------------------------- wave ops
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Inclusive prefix sum over the 64 lanes (all lanes active).
-__device__ __forceinline__ int wave_scan_add(int x)
-{
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
-    return x;
-}
-
-// Inclusive prefix max over the 64 lanes of non-negative values.
-__device__ __forceinline__ int wave_scan_max(int x)
-{
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
-    return x;
-}
-
-__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
-
-__device__ __forceinline__ int shfl_idx(int x, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, x); }
-__device__ __forceinline__ int64_t shfl_idx(int64_t x, int src)
-{
-    int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(x & 0xffffffff));
-    int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(x >> 32));
-    return ((int64_t)hi << 32) | (unsigned int)lo;
-}
-__device__ __forceinline__ int readlane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
-__device__ __forceinline__ int64_t readlane(int64_t x, int l)
-{
-    int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffff), l);
-    int hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
-    return ((int64_t)hi << 32) | (unsigned int)lo;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
-{
-    for (int off = 32; off > 0; off >>= 1)
-        x += __shfl_xor(x, off);
-    return x;
-}
 
 // Per-wave attribution of triangles to the graphs of a batch.  The state
 // sits in LDS, in the kMarkStride - kWave words after the wave's mark array
@@ -549,16 +494,16 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
         for (int i = threadIdx.x; i < size; i += blockDim.x)
             slot_clear<kGlobal>(tab + i);
     if (threadIdx.x == 0) {
-        misc[0] = 0;                 // window-unit cursor
-        misc[3] = a.force_linear;    // linear-probing table for this vertex
+        misc[0] = 0;                       // window-unit cursor
+        misc[3] = a.force_linear == 1;     // linear-probing table for this vertex
     }
     if constexpr (kGlobal)
         __threadfence();  // HBM slots: cleared values visible in L2 before any insert/lookup
     __syncthreads();
     bool linear = false;
     if constexpr (kTable) {
-        bool failed = false;
-        if (!a.force_linear)
+        bool failed = a.force_linear == 2;  // forced: rebuild after the cuckoo attempt
+        if (a.force_linear != 1)
             for (int i = threadIdx.x; i < dH; i += blockDim.x)
                 failed |= cuckoo_insert(tab, lg - 1, a.adjH[sH + (OffT)i]);
         if (failed)
@@ -568,6 +513,8 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
         __syncthreads();
         linear = misc[3] != 0;
         if (linear) {  // rebuild this vertex's table with linear probing
+            if (threadIdx.x == 0 && a.force_linear != 1)
+                atomicAdd(&a.ctl->rebuilds, 1ull);
             __syncthreads();
             for (int i = threadIdx.x; i < size; i += blockDim.x)
                 slot_clear<kGlobal>(tab + i);
@@ -703,8 +650,9 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
         for (int i = lane; i < size; i += kWave)
             tab[i] = kEmpty;
         wave_sync();
+        // force_linear 1: linear probing only; 2: rebuild after the cuckoo attempt
         bool failed = a.force_linear != 0;
-        if (!a.force_linear) {
+        if (a.force_linear != 1) {
             if (lane < dH)
                 failed |= cuckoo_insert(tab, lg - 1, head);
             for (int i = lane + kWave; i < dH; i += kWave)
@@ -713,6 +661,8 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
         wave_sync();
         linear = __ballot(failed) != 0;
         if (linear) {  // rebuild this vertex's table with linear probing
+            if (lane == 0 && a.force_linear != 1)
+                atomicAdd(&a.ctl->rebuilds, 1ull);
             for (int i = lane; i < size; i += kWave)
                 tab[i] = kEmpty;
             wave_sync();
@@ -1107,98 +1057,6 @@ __global__ void __launch_bounds__(256) side_pass_kernel(const OffT *__restrict__
     }
 }
 
-// Exclusive scan of kept[i] + incnt[i], i < V, into begX[0, V] in three
-// launches: per-block sums, one workgroup scanning those, per-block scans.
-__global__ void __launch_bounds__(1024) side_scan_reduce_kernel(const int *__restrict__ incnt,
-                                                                const int *__restrict__ kept,
-                                                                long long V,
-                                                                long long *__restrict__ bsum)
-{
-    __shared__ long long part[1024 / kWave];
-    const long long lo = (long long)blockIdx.x * kSideScanPer;
-    const long long hi = min(lo + kSideScanPer, V);
-    long long x = 0;
-    for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x)
-        x += (long long)incnt[i] + kept[i];
-    x = (long long)wave_sum((unsigned long long)x);
-    if (lane_id() == 0)
-        part[threadIdx.x / kWave] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long t = 0;
-        for (int w = 0; w < (int)(blockDim.x / kWave); w++)
-            t += part[w];
-        bsum[blockIdx.x] = t;
-    }
-}
-
-template <typename OffT>
-__global__ void __launch_bounds__(1024) side_scan_top_kernel(long long *__restrict__ bsum,
-                                                             long long nb, long long V,
-                                                             OffT *__restrict__ begX)
-{
-    __shared__ long long part[1024];
-    __shared__ long long base;
-    if (threadIdx.x == 0)
-        base = 0;
-    __syncthreads();
-    for (long long c0 = 0; c0 < nb; c0 += blockDim.x) {
-        const long long i = c0 + threadIdx.x;
-        const long long x = i < nb ? bsum[i] : 0;
-        part[threadIdx.x] = x;
-        __syncthreads();
-        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-            const long long y = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-            __syncthreads();
-            part[threadIdx.x] += y;
-            __syncthreads();
-        }
-        if (i < nb)
-            bsum[i] = base + part[threadIdx.x] - x;
-        __syncthreads();
-        if (threadIdx.x == blockDim.x - 1)
-            base += part[threadIdx.x];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        begX[V] = (OffT)base;
-}
-
-template <typename OffT>
-__global__ void __launch_bounds__(1024) side_scan_out_kernel(const int *__restrict__ incnt,
-                                                             const int *__restrict__ kept,
-                                                             long long V,
-                                                             const long long *__restrict__ bsum,
-                                                             OffT *__restrict__ begX)
-{
-    // thread t scans elements [lo + 8t, lo + 8t + 8)
-    constexpr int kPer = kSideScanPer / 1024;
-    __shared__ long long part[1024];
-    const long long lo = (long long)blockIdx.x * kSideScanPer + (long long)threadIdx.x * kPer;
-    int x[kPer];
-    long long t = 0;
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        x[k] = lo + k < V ? incnt[lo + k] + kept[lo + k] : 0;
-        t += x[k];
-    }
-    part[threadIdx.x] = t;
-    __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-        const long long y = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += y;
-        __syncthreads();
-    }
-    long long run = bsum[blockIdx.x] + part[threadIdx.x] - t;
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        if (lo + k < V)
-            begX[lo + k] = (OffT)run;
-        run += x[k];
-    }
-}
-
 // rowX[at] = the row whose entries [begX[u], begX[u+1]) hold position at.
 // A wave takes 64 consecutive rows, whose entries are contiguous, and
 // writes them in 64-wide windows: each non-empty row marks its first
@@ -1239,13 +1097,12 @@ __global__ void __launch_bounds__(256) side_rows_kernel(const OffT *__restrict__
 
 template <typename OffT>
 hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, int *cnt,
-                             long long *bsum, OffT *begX, WalkEntry<OffT> *entX, int32_t *rowX,
+                             long long *bsum, OffT *begX, WalkEntry<OffT> *entX,
                              hipStream_t stream)
 {
     if (V <= 0)
         return hipSuccess;
     int *incnt = cnt, *kept = cnt + V;
-    const long long nb = side_scan_blocks(V);
     const int pass_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
     hipError_t e = launch_fill_u32((uint32_t *)incnt, 0u, V, stream);
     if (e != hipSuccess)
@@ -1253,29 +1110,33 @@ hipError_t launch_side_build(const OffT *beg, const int32_t *adj, long long V, i
     hipLaunchKernelGGL((side_pass_kernel<OffT, 1>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, incnt, kept, (const OffT *)nullptr,
                        (WalkEntry<OffT> *)nullptr, (unsigned long long *)nullptr);
-    hipLaunchKernelGGL(side_scan_reduce_kernel, dim3((unsigned)nb), dim3(1024), 0, stream,
-                       (const int *)incnt, (const int *)kept, V, bsum);
-    hipLaunchKernelGGL(side_scan_top_kernel<OffT>, dim3(1), dim3(1024), 0, stream, bsum, nb, V,
-                       begX);
-    hipLaunchKernelGGL(side_scan_out_kernel<OffT>, dim3((unsigned)nb), dim3(1024), 0, stream,
-                       (const int *)incnt, (const int *)kept, V, (const long long *)bsum, begX);
+    if ((e = launch_scan<OffT>(incnt, kept, V, bsum, begX, stream)) != hipSuccess)
+        return e;
     hipLaunchKernelGGL((side_pass_kernel<OffT, 2>), dim3(pass_blocks), dim3(256), 0, stream,
                        beg, adj, V, incnt, (int *)nullptr, (const OffT *)begX, entX,
                        (unsigned long long *)nullptr);
-    {
-        const int row_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
-        hipLaunchKernelGGL((side_rows_kernel<OffT>), dim3(row_blocks), dim3(256), 0, stream,
-                           (const OffT *)begX, V, rowX);
-    }
+    return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_side_rows(const OffT *begX, long long V, int32_t *rowX, hipStream_t stream)
+{
+    if (V <= 0)
+        return hipSuccess;
+    const int row_blocks = (int)min((V + 4 * kWave - 1) / (4 * kWave), 16384ll);
+    hipLaunchKernelGGL((side_rows_kernel<OffT>), dim3(row_blocks), dim3(256), 0, stream, begX, V,
+                       rowX);
     return hipGetLastError();
 }
 
 template hipError_t launch_side_build<int32_t>(const int32_t *, const int32_t *, long long, int *,
                                                long long *, int32_t *, WalkEntry<int32_t> *,
-                                               int32_t *, hipStream_t);
+                                               hipStream_t);
 template hipError_t launch_side_build<int64_t>(const int64_t *, const int32_t *, long long, int *,
                                                long long *, int64_t *, WalkEntry<int64_t> *,
-                                               int32_t *, hipStream_t);
+                                               hipStream_t);
+template hipError_t launch_side_rows<int32_t>(const int32_t *, long long, int32_t *, hipStream_t);
+template hipError_t launch_side_rows<int64_t>(const int64_t *, long long, int32_t *, hipStream_t);
 
 template <typename OffT>
 hipError_t launch_side_stats(const OffT *beg, const int32_t *adj, long long V,

@@ -35,6 +35,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <memory>
 #include <cstdlib>
@@ -128,11 +129,15 @@ struct PairBlocks {
 // that do not start with a digit are skipped, a line gives the first two
 // integers on it (a third, the 1.mmio weight column, is ignored), and with
 // `header` the first such line is "rows cols entries" (hv[0], hv[1]).  Each
-// pair goes to push(u, v).
+// pair goes to push(u, v).  An id below min_id (the reference's readers index
+// arrays with it: undefined behaviour) is an error: *bad_id holds it and the
+// call returns false (*bad_id stays LLONG_MIN on other failures).
 template <typename Push>
-static bool read_pairs(const std::string &path, Push &&push, bool header, long long *hv)
+static bool read_pairs(const std::string &path, Push &&push, bool header, long long *hv,
+                       long long min_id, long long *bad_id)
 {
     ByteReader r(path);
+    *bad_id = LLONG_MIN;
     if (!r.ok())
         return false;
     bool first = true;
@@ -168,6 +173,10 @@ static bool read_pairs(const std::string &path, Push &&push, bool header, long l
             hv[1] = v[1];
             first = false;
         } else if (nv >= 2) {
+            if (v[0] < min_id || v[1] < min_id) {
+                *bad_id = v[0] < min_id ? v[0] : v[1];
+                return false;
+            }
             push((int32_t)v[0], (int32_t)v[1]);
         }
     }
@@ -341,8 +350,13 @@ static int cmd_undirect(int argc, char **argv)
         return 2;
     std::string out = argc > 1 ? argv[1] : "1.mmio";
     PairBlocks pb;
-    if (!read_pairs(argv[0], [&](int32_t u, int32_t v) { pb.push(u, v); }, false, nullptr)) {
-        fprintf(stderr, "trust_tool: cannot read %s\n", argv[0]);
+    long long bad = LLONG_MIN;
+    if (!read_pairs(argv[0], [&](int32_t u, int32_t v) { pb.push(u, v); }, false, nullptr, 0,
+                    &bad)) {
+        if (bad != LLONG_MIN)
+            fprintf(stderr, "trust_tool: %s: negative vertex id %lld\n", argv[0], bad);
+        else
+            fprintf(stderr, "trust_tool: cannot read %s\n", argv[0]);
         return 1;
     }
     int32_t maxid = -1;
@@ -465,16 +479,20 @@ static int cmd_preprocess(int argc, char **argv)
     std::string in = pos.size() > 0 ? pos[0] : "1.mmio";
     std::string outdir = pos.size() > 1 ? pos[1] : ".";
     std::vector<int32_t> a, b;
-    long long hv[2] = {-1, -1};
+    long long hv[2] = {-1, -1}, bad = LLONG_MIN;
     if (!read_pairs(
             in,
             [&](int32_t u, int32_t v) {
                 a.push_back(u);
                 b.push_back(v);
             },
-            true, hv) ||
+            true, hv, 1, &bad) ||
         hv[0] < 0 || hv[1] < 0) {
-        fprintf(stderr, "trust_tool: cannot read %s\n", in.c_str());
+        if (bad != LLONG_MIN)
+            fprintf(stderr, "trust_tool: %s: vertex id %lld (1.mmio ids start at 1)\n",
+                    in.c_str(), bad);
+        else
+            fprintf(stderr, "trust_tool: cannot read %s\n", in.c_str());
         return 1;
     }
     for (size_t e = 0; e < a.size(); e++) {  // 1-based ids in the file
