@@ -189,6 +189,24 @@ int trust_triangle_count_partition(int rank, const char *folder, int total_proce
 int trust_count_csr(const int64_t *beg, const int32_t *adj, int64_t vertices, int64_t edges,
                     const trust_params *p, trust_result *r);
 
+/* No counterpart in the reference.  trust_count_csr for a CSR the caller
+ * already holds in DEVICE memory (beg_dev: vertices+1 int64 offsets in the
+ * begin.bin layout, adj_dev: edges int32 ids), on the caller's HIP stream
+ * (null: the default stream), that allocates nothing: no hipMalloc, hipFree
+ * or allocating library call.  For callers that cannot allocate (code under
+ * stream capture, serving loops, harnesses that forbid it).  Its scratch is
+ * one fixed device array of the library (about 34 MB per GPU, held until
+ * the process ends).  Limits: vertices <= 2^22, edges < 2^30, out-degrees
+ * whose tables fit the LDS (about 16k at the default launch), the
+ * reference walk with offset lookups, the degree split; FORCE_SPILL,
+ * OFFSETS64, HIP_GRAPH, SHORT_WALK, FRESH_INDEX and the learned schedule are
+ * rejected.  The CSR is checked on the device (offsets, ids) unless
+ * TRUST_FLAG_NO_VALIDATE.  Calls on one GPU share its scratch: calls that
+ * overlap on one GPU race.  seconds = device wall clock of plan + count. */
+int trust_count_csr_noalloc(const int64_t *beg_dev, const int32_t *adj_dev, int64_t vertices,
+                            int64_t edges, const trust_params *p, trust_result *r,
+                            void *stream);
+
 /*
  * Sum over all n^3 (i,j,k) jobs of a 2D hash partition held in host memory,
  * streamed through one GPU with asynchronous tile copies.  tiles[i*n+j] is

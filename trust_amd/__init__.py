@@ -102,6 +102,9 @@ SIGNATURES = {
                                   C.POINTER(Result)]),
     "trust_count_tiles": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Params),
                                     C.POINTER(Result)]),
+    # device pointers (beg int64, adj int32), sizes, params, result, HIP stream
+    "trust_count_csr_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                                          C.POINTER(Params), C.POINTER(Result), C.c_void_p]),
     "trust_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "trust_ctx_destroy": (None, [C.c_void_p]),
     "trust_ctx_load_csr": (C.c_int, [C.c_void_p, _P64, _P32, C.c_int64, C.c_int64, C.c_uint32,
