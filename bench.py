@@ -261,8 +261,12 @@ def bench_rmat(args, T):
         line["config"] = {"workload": f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}",
                           "V": V, "E": E, "triangles": r0.triangles, "W_probes": r0.probes,
                           "only_head": True, "index_ms": head["idx_s"] * 1e3,
-                          "count_kernel_ms": head["cnt_s"] * 1e3}
+                          "count_kernel_ms": head["cnt_s"] * 1e3,
+                          "alg_bytes_per_count": r0.alg_bytes}
         ctx.close()
+        # one 2 GiB copy: the known-traffic kernel that calibrates byte counters
+        line["config"]["copy_bytes"] = 2 << 30
+        line["config"]["copy_s"] = T.membench_copy(2 << 30, reps=1)
         return line
     deg = run_steps(lambda: ctx.count(**base_kw, schedule=T.SCHED_DEGREE), args.warmup,
                     args.steps, "degree split, index kept", E)
@@ -596,6 +600,10 @@ def bench_tiles(args, T):
     fixture = check_fixture(f"rmat{args.scale}", beg, adj, whole["r"].triangles) \
         if args.edge_factor == 16 and args.seed == 1 else None
     part = run_steps(lambda: T.count_tiles(tiles, n), 1, args.steps, f"{n}x{n} tiles", E)
+    rt = part["r"]
+    log(f"tiles: stream {rt.seconds * 1e3:.1f} ms = counting launches {rt.count_seconds * 1e3:.1f} ms"
+        f" + tile walk builds {rt.index_seconds * 1e3:.1f} ms + copies not hidden; host time in "
+        f"uploads {rt.h2d_seconds * 1e3:.1f} ms")
     if part["r"].triangles != whole["r"].triangles:
         raise SystemExit(f"partitioned {part['r'].triangles} != unpartitioned "
                          f"{whole['r'].triangles}")
@@ -603,12 +611,22 @@ def bench_tiles(args, T):
     workload = f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}_tiles{n}x{n}"
     tile_bytes = sum(b.nbytes + a.nbytes for b, a in tiles)
     line = line_head(args, "tiles", E / dev_s / 1e9, "GTEPS", dev_s * 1e3)
+    rt = part["r"]
     line["config"] = {
-        "workload": workload, "V": V, "E": E, "triangles": part["r"].triangles,
+        "workload": workload, "V": V, "E": E, "triangles": rt.triangles,
         "oracle_parity": fixture,
         "jobs": n ** 3, "tile_bytes": tile_bytes, "parallelism": "single-gpu",
+        "walk": "short-side (global out-degree)" if rt.walk_used else "reference",
+        "W_probes": rt.probes,
         "unpartitioned_gteps": E / whole["dev_s"] / 1e9,
-        "timing": "streamed count incl. asynchronous host-to-device tile copies",
+        "timing": "streamed count incl. asynchronous host-to-device tile copies and the tile "
+                  "walks' builds",
+        "count_launches_ms": rt.count_seconds * 1e3,
+        "tile_walk_builds_ms": rt.index_seconds * 1e3,
+        "host_upload_ms": rt.h2d_seconds * 1e3,
+        "overlap_note": "the stream's time against its parts: counting launches, walk builds "
+                        "(synchronous) and host time inside pageable tile uploads, which overlap "
+                        "the counting of the previous job",
     }
     r0 = part["r"]
     achieved = r0.alg_bytes / dev_s / 1e9

@@ -179,6 +179,21 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
     if (rc)
         return rc;
 
+    // per job: events around its counting launches (the count's share of the
+    // streamed time); host time inside the tile uploads; the tile walks' builds
+    double copy_host_s = 0, build_s = 0;
+    std::vector<hipEvent_t> jev((size_t)(2 * njobs), nullptr);
+    struct EventGuard {
+        std::vector<hipEvent_t> &e;
+        ~EventGuard()
+        {
+            for (auto x : e)
+                if (x)
+                    (void)hipEventDestroy(x);
+        }
+    } jev_guard{jev};
+    for (auto &e : jev)
+        HIPCHK(hipEventCreate(&e));
     auto job_tiles = [n, short_walk](long long job, int out[3]) {
         int i = (int)(job / ((long long)n * n)), j = (int)(job / n % n), k = (int)(job % n);
         out[0] = i * n + j;  // hash (the in-walk's probe)
@@ -208,10 +223,13 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         if (v.last_job >= 0)
             HIPCHK(hipStreamWaitEvent(ctx->copy_stream, v.released, 0));
         const trust_csr &src = tiles[t];
+        const auto c0 = std::chrono::steady_clock::now();
         int rc2 = upload_csr(ctx, v.g, src.beg, src.adj, src.vertices, src.edges, off64,
                              ctx->copy_stream);
         if (rc2)
             return rc2;
+        // pageable host memory: the call returns once the runtime has staged the bytes
+        copy_host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
         HIPCHK(hipEventRecord(v.ready, ctx->copy_stream));
         v.tile = t;
         v.last_job = job;
@@ -244,6 +262,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
             HIPCHK(hipStreamWaitEvent(ctx->stream, slots[ws].ready, 0));
             const DevCsr &wg = slots[ws].g;
             const int32_t *dg = (const int32_t *)degs.p;
+            const auto b0 = std::chrono::steady_clock::now();
             rc = off64 ? tile_walk_build<int64_t>(ctx, (const int64_t *)wg.beg.p,
                                                   (const int32_t *)wg.adj.p, rows_of[ji],
                                                   rows_of[jk], dg + deg_off[ji], dg + deg_off[jk],
@@ -254,6 +273,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                                                   walks[(size_t)jk]);
             if (rc)
                 return rc;
+            build_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - b0).count();
             HIPCHK(hipEventRecord(slots[ws].released, ctx->stream));
             walk_row[(size_t)jk] = ji;
         }
@@ -273,6 +293,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
             HIPCHK(hipStreamWaitEvent(ctx->stream, slots[sl[q]].ready, 0));
         }
         const DevCsr &th = slots[sl[0]].g, &tp = slots[sl[2]].g;
+        HIPCHK(hipEventRecord(jev[(size_t)(2 * job)], ctx->stream));
         if (short_walk) {
             const TileWalk &w = walks[(size_t)jk];
             CsrViews vo{th.beg.p, (const int32_t *)th.adj.p, w.out_beg.p,
@@ -291,6 +312,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                 return rc;
         }
         (void)jj;
+        HIPCHK(hipEventRecord(jev[(size_t)(2 * job + 1)], ctx->stream));
         for (int q = 0; q < 3; q++)
             if (sl[q] >= 0)
                 HIPCHK(hipEventRecord(slots[sl[q]].released, ctx->stream));
@@ -325,6 +347,12 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    double count_s = 0;
+    for (long long j = 0; j < njobs; j++) {
+        float jm = 0;
+        HIPCHK(hipEventElapsedTime(&jm, jev[(size_t)(2 * j)], jev[(size_t)(2 * j + 1)]));
+        count_s += jm * 1e-3;
+    }
     uint64_t total = 0, probes = 0, rebuilds = 0;
     for (long long j = 0; j < njobs; j++) {
         const TcCtl &x = h[(size_t)(short_walk ? 2 * j : j)];
@@ -340,8 +368,10 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         r->edges = Esum;
         r->triangles = total;
         r->probes = probes;
-        r->seconds = ms * 1e-3;
-        r->count_seconds = r->seconds;
+        r->seconds = ms * 1e-3;       // the whole stream: copies, walk builds, counts
+        r->count_seconds = count_s;   // the jobs' counting launches alone
+        r->index_seconds = build_s;   // the tile walks' builds
+        r->h2d_seconds = copy_host_s; // host time inside the pageable tile uploads
         r->gteps = r->seconds > 0 ? (double)Esum / r->seconds / 1e9 : 0;
         r->alg_bytes = alg_bytes(Vsum, Esum, probes, off64);
         r->threads = L.main.threads;

@@ -54,8 +54,7 @@ constexpr int kWin = 4;                // 64-wide windows per step of passes A a
 constexpr int kPassThreads = 1024;
 constexpr int kPassWaves = kPassThreads / kWave;
 constexpr int kRankThreads = 1024;
-constexpr int kPlaceThreads = 256;
-constexpr long long kChunk = 16384;     // records per C2 chunk
+constexpr int kPlaceThreads = 1024;
 constexpr int kScanPer = 8192;         // elements per scan workgroup (1024 threads x 8)
 
 // An in-entry on its way to row v: the list it probes is adj[start, start + deg).
@@ -108,6 +107,9 @@ struct WiPassArgs {
     int *kept;                    // A: out-entries per row
     int *M;                       // A: in-entries per bin and workgroup, bin-major
     unsigned long long *stats;    // A: probes of both walks (resident graph)
+    // resident graph: d+ of each entry's vertex (edges of rows with d+ >= 2),
+    // written by A, read by B in place of a second pair of offset lookups
+    int32_t *dvE;
     const long long *O;           // B: each workgroup's first record in each bin
     const OffT *outBeg;           // B: each row's first out-entry in tmpOut
     typename WiFmt<OffT, kTile>::Out *tmpOut;
@@ -164,13 +166,16 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
             if (d > 0 && (long long)s >= p && (long long)s < p + kWave * kWin)
                 mark[(int)((long long)s - p)] = lane;
             wave_sync();
-            int own[kWin], v[kWin];
+            int own[kWin], v[kWin], dvk[kWin];
 #pragma unroll
             for (int k = 0; k < kWin; k++) {
                 own[k] = max(wave_scan_max(mark[k * kWave + lane]), carry);
                 carry = readlane(own[k], 63);
                 const long long pos = p + k * kWave + lane;
                 v[k] = pos < e1 ? adj[pos] : -1;
+                dvk[k] = 0;
+                if constexpr (kPass == 1 && !kTile)
+                    dvk[k] = pos < e1 ? a.dvE[pos] : 0;
             }
             wave_sync();
 #pragma unroll
@@ -190,14 +195,17 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
                     int dv;
                     if constexpr (kTile) {
                         dv = a.degC[v[k]];
-                    } else {
+                    } else if constexpr (kPass == 0) {
                         bv = beg[v[k]];
                         dv = (int)(beg[v[k] + 1] - bv);
+                    } else {
+                        dv = dvk[k];  // an out-entry looks up its list start below
                     }
                     if constexpr (kPass == 0) {
                         if constexpr (!kTile) {
                             w_ref += (unsigned long long)dv;
                             w_short += (unsigned long long)min(du, dv);
+                            a.dvE[p + k * kWave + lane] = dv;
                         }
                         if (dv <= du)
                             atomicAdd(rowcnt + own[k], 1);
@@ -209,7 +217,7 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
                             if constexpr (kTile)
                                 a.tmpOut[obu + (OffT)slot] = v[k];
                             else
-                                a.tmpOut[obu + (OffT)slot] = WalkEntry<OffT>{bv, dv};
+                                a.tmpOut[obu + (OffT)slot] = WalkEntry<OffT>{beg[v[k]], dv};
                         } else {
                             const CurT at = atomicAdd(bins + (v[k] >> kBinLg), (CurT)1);
                             if constexpr (kTile)
@@ -254,9 +262,15 @@ wi_rank_kernel(const WiSlice *__restrict__ sl, const Rec *__restrict__ recs,
     for (int t = threadIdx.x; t < kBinSize; t += blockDim.x)
         cnt[t] = 0;
     __syncthreads();
-    for (long long i = s.lo + threadIdx.x; i < s.hi; i += blockDim.x)
-        rel[i] = atomicAdd(&cnt[recs[i].v & (kBinSize - 1)], 1);
-    __syncthreads();
+    // one round of blockDim records at a time: a target's ranks in round r all
+    // precede its ranks in round r + 1, so the records of any whole rounds
+    // (C2's chunks) hold a contiguous rank range of each target
+    for (long long r0 = s.lo; r0 < s.hi; r0 += blockDim.x) {
+        const long long i = r0 + threadIdx.x;
+        if (i < s.hi)
+            rel[i] = atomicAdd(&cnt[recs[i].v & (kBinSize - 1)], 1);
+        __syncthreads();
+    }
     const long long v0 = (long long)s.bin << kBinLg;
     for (int t = threadIdx.x; t < kBinSize; t += blockDim.x) {
         const int c = cnt[t];
@@ -273,18 +287,33 @@ wi_rank_kernel(const WiSlice *__restrict__ sl, const Rec *__restrict__ recs,
 // kept[v] (the row's out-part comes first) for the resident graph, at
 // begX[v] for a tile walk, whose in-rows are a CSR of their own.
 //
-// The stores land anywhere in the bin's rows, one 8-byte entry at a time:
-// spread over the whole chip they are random stores, about 20 G/s (39 ms at
-// R-MAT scale 26).  So the records are cut into chunks, the bins are dealt
-// to the 8 XCDs (largest first, to the least loaded), and a workgroup takes
-// chunks from the queue of the XCD it runs on (hardware XCC_ID), then helps
-// the others.  The workgroups of one XCD then place one or two bins at a
-// time, whose rows (a few MB) stay in that XCD's L2 until their lines are
-// full.  The hardware register only steers locality: any workgroup may
-// take any chunk, so counts do not depend on it.
+// Stored in record order, the slots land anywhere in the bin's rows, one
+// 8-byte entry at a time: random stores at about 20 G/s (39 ms at R-MAT
+// scale 26).  So a workgroup takes a chunk of whole C1 rounds, whose records
+// of any one target hold consecutive slots, sorts them by target in LDS (a
+// counting sort over the bin's 16k targets) and stores them in that order:
+// the records of a target go out together, to consecutive addresses.  The
+// chunks of a bin are queued on one XCD (bins dealt largest first to the
+// least loaded XCD); a workgroup reads the hardware XCC_ID and drains its
+// own XCD's queue first, then the others', so one XCD's L2 holds the lines
+// a bin's concurrent chunks fill.  The register only steers locality: any
+// workgroup may take any chunk.
 __device__ __forceinline__ int xcc_id()
 {
     return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1));  // HW_REG_XCC_ID
+}
+
+template <typename OffT, bool kTile>
+struct PlaceStage {
+    OffT slot;
+    typename WiFmt<OffT, kTile>::Out val;
+};
+template <typename OffT, bool kTile>
+constexpr int place_chunk()
+{
+    // LDS: 16k counters (64 KB) + one chunk of staged records (<= 64 KB)
+    return sizeof(PlaceStage<OffT, kTile>) <= 8 ? 8192
+           : sizeof(PlaceStage<OffT, kTile>) <= 16 ? 4096 : 2048;
 }
 
 template <typename OffT, bool kTile>
@@ -295,8 +324,16 @@ wi_place_kernel(const WiSlice *__restrict__ sl, const WiChunk *__restrict__ chun
                 const OffT *__restrict__ begX, const int *__restrict__ kept,
                 typename WiFmt<OffT, kTile>::Out *__restrict__ entX)
 {
+    using Out = typename WiFmt<OffT, kTile>::Out;
+    constexpr int kC = place_chunk<OffT, kTile>();
+    constexpr int kPer = kC / kPlaceThreads;  // records per thread
+    constexpr int kCntPer = kBinSize / kPlaceThreads;
+    __shared__ int pos[kBinSize];
+    __shared__ PlaceStage<OffT, kTile> stage[kC];
     __shared__ int take;
+    __shared__ int part[kPlaceThreads / kWave];
     const int home = xcc_id();
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
     for (int x = 0; x < kXcds;) {
         const int qx = (home + x) & (kXcds - 1);
         __syncthreads();
@@ -305,6 +342,8 @@ wi_place_kernel(const WiSlice *__restrict__ sl, const WiChunk *__restrict__ chun
             const int c = (int)atomicAdd(&q->next[qx * 32], 1u);
             take = c < n ? q->begin[qx] + c : -1;
         }
+        for (int k = 0; k < kCntPer; k++)
+            pos[threadIdx.x * kCntPer + k] = 0;
         __syncthreads();
         const int c = take;
         if (c < 0) {
@@ -314,18 +353,60 @@ wi_place_kernel(const WiSlice *__restrict__ sl, const WiChunk *__restrict__ chun
         const WiChunk ch = chunks[c];
         const WiSlice s = sl[ch.slice];
         const int *sb = s.multi >= 0 ? sbase + (long long)s.multi * kBinSize : nullptr;
-        for (long long i = ch.lo + threadIdx.x; i < ch.hi; i += blockDim.x) {
-            const auto r = recs[i];
-            OffT at = begX[r.v] + (OffT)rel[i];
-            if constexpr (!kTile)
-                at += (OffT)kept[r.v];
-            if (sb)
-                at += (OffT)sb[r.v & (kBinSize - 1)];
-            if constexpr (kTile)
-                entX[at] = r.id;
-            else
-                entX[at] = WalkEntry<OffT>{r.start, r.deg};
+        const int n = (int)(ch.hi - ch.lo);
+        OffT slot[kPer];
+        Out val[kPer];
+        int tgt[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int j = k * kPlaceThreads + threadIdx.x;
+            tgt[k] = -1;
+            if (j < n) {
+                const long long i = ch.lo + j;
+                const auto r = recs[i];
+                OffT at = begX[r.v] + (OffT)rel[i];
+                if constexpr (!kTile)
+                    at += (OffT)kept[r.v];
+                tgt[k] = r.v & (kBinSize - 1);
+                if (sb)
+                    at += (OffT)sb[tgt[k]];
+                slot[k] = at;
+                if constexpr (kTile)
+                    val[k] = r.id;
+                else
+                    val[k] = WalkEntry<OffT>{r.start, r.deg};
+                atomicAdd(&pos[tgt[k]], 1);
+            }
         }
+        __syncthreads();
+        // exclusive scan of the 16k counts: each thread its kCntPer, then the
+        // per-thread totals across the workgroup
+        int run = 0, mine[kCntPer];
+#pragma unroll
+        for (int k = 0; k < kCntPer; k++) {
+            mine[k] = pos[threadIdx.x * kCntPer + k];
+            run += mine[k];
+        }
+        const int incl = wave_scan_add(run);
+        if (lane == kWave - 1)
+            part[wave] = incl;
+        __syncthreads();
+        int before = incl - run;
+        for (int w = 0; w < wave; w++)
+            before += part[w];
+#pragma unroll
+        for (int k = 0; k < kCntPer; k++) {
+            pos[threadIdx.x * kCntPer + k] = before;
+            before += mine[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; k++)
+            if (tgt[k] >= 0)
+                stage[atomicAdd(&pos[tgt[k]], 1)] = PlaceStage<OffT, kTile>{slot[k], val[k]};
+        __syncthreads();
+        for (int j = threadIdx.x; j < n; j += kPlaceThreads)
+            entX[stage[j].slot] = stage[j].val;
     }
 }
 
@@ -502,7 +583,7 @@ bool walk_index_binned(const Ctx *ctx) { return ctx->g.V > 0 && walk_bins(ctx->g
 template <typename OffT, bool kTile>
 static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long rows, long long cols,
                     const int32_t *degR, const int32_t *degC, int *kept, OffT *outBeg,
-                    unsigned long long stats_out[2])
+                    unsigned long long stats_out[2], int32_t *dvE)
 {
     WalkIndexState &w = ctx->wi;
     const int nbins = std::max(1, walk_bins(cols));
@@ -537,6 +618,7 @@ static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
     pa.kept = kept;
     pa.M = (int *)w.M.p;
     pa.stats = st;
+    pa.dvE = dvE;
     if (rows > 0) {
         hipLaunchKernelGGL((wi_pass_kernel<OffT, 0, kTile>), dim3(G), dim3(kPassThreads), lds,
                            ctx->stream, pa);
@@ -578,7 +660,7 @@ template <typename OffT, bool kTile>
 static int wi_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long rows, long long cols,
                     const int32_t *degR, const int32_t *degC, const int *kept, const OffT *outBeg,
                     int *incnt, typename WiFmt<OffT, kTile>::Out *tmpOut, OffT *begX,
-                    typename WiFmt<OffT, kTile>::Out *entX)
+                    typename WiFmt<OffT, kTile>::Out *entX, int32_t *dvE)
 {
     using Rec = typename WiFmt<OffT, kTile>::Rec;
     WalkIndexState &w = ctx->wi;
@@ -625,6 +707,9 @@ static int wi_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
         bin_xcd[(size_t)b] = best;
         load[best] += w.bin_start_h[(size_t)b + 1] - w.bin_start_h[(size_t)b];
     }
+    // C2 chunks: whole C1 rounds, so that their ranks of a target are contiguous
+    constexpr long long kC = place_chunk<OffT, kTile>();
+    static_assert(kC % kRankThreads == 0, "a C2 chunk must hold whole C1 rounds");
     std::vector<WiChunk> &ch = w.chunks_h;
     ch.clear();
     WiQueue qh;
@@ -634,8 +719,8 @@ static int wi_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
         for (size_t k = 0; k < sl.size(); k++) {
             if (bin_xcd[(size_t)sl[k].bin] != x)
                 continue;
-            for (long long lo = sl[k].lo; lo < sl[k].hi; lo += kChunk)
-                ch.push_back(WiChunk{(int)k, 0, lo, std::min(sl[k].hi, lo + kChunk)});
+            for (long long lo = sl[k].lo; lo < sl[k].hi; lo += kC)
+                ch.push_back(WiChunk{(int)k, 0, lo, std::min(sl[k].hi, lo + kC)});
         }
     }
     qh.begin[kXcds] = (int)ch.size();
@@ -667,6 +752,7 @@ static int wi_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
     pa.degC = degC;
     pa.O = (const long long *)w.O.p;
     pa.outBeg = outBeg;
+    pa.dvE = dvE;
     pa.tmpOut = tmpOut;
     pa.recs = (Rec *)w.recs.p;
     if (rows > 0) {
@@ -684,8 +770,8 @@ static int wi_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
     HIPCHK(launch_scan<OffT>(kTile ? incnt : kept, kTile ? nullptr : incnt, cols,
                              (long long *)w.bsum.p, begX, ctx->stream));
     if (!sl.empty()) {
-        // persistent: 8 workgroups of 256 per CU (none waits on another)
-        const int blocks = (int)std::min<long long>((long long)ch.size(), 8ll * ctx->num_cus);
+        // persistent: one workgroup per CU (the LDS of a chunk; none waits on another)
+        const int blocks = (int)std::min<long long>((long long)ch.size(), (long long)ctx->num_cus);
         hipLaunchKernelGGL((wi_place_kernel<OffT, kTile>), dim3(blocks), dim3(kPlaceThreads), 0,
                            ctx->stream, (const WiSlice *)w.slices.p, (const WiChunk *)chd, qd,
                            (const Rec *)w.recs.p, (const int *)w.rel.p, (const int *)w.sbase.p,
@@ -712,11 +798,13 @@ static int count_pass_t(Ctx *ctx)
     const long long V = g.V;
     int rc;
     if ((rc = w.out_beg.ensure(sizeof(OffT) * (size_t)(V + 1))) ||
+        (rc = w.dv.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(g.E, 1))) ||
         (rc = ctx->side_cnt.ensure(2 * sizeof(int) * (size_t)V)))
         return rc;
     unsigned long long h[2];
     rc = wi_count<OffT, false>(ctx, (const OffT *)g.beg.p, (const int32_t *)g.adj.p, V, V, nullptr,
-                               nullptr, (int *)ctx->side_cnt.p + V, (OffT *)w.out_beg.p, h);
+                               nullptr, (int *)ctx->side_cnt.p + V, (OffT *)w.out_beg.p, h,
+                               (int32_t *)w.dv.p);
     if (rc)
         return rc;
     ctx->side_w_ref = h[0];
@@ -748,7 +836,7 @@ static int build_t(Ctx *ctx)
     rc = wi_build<OffT, false>(ctx, (const OffT *)g.beg.p, (const int32_t *)g.adj.p, V, V, nullptr,
                                nullptr, kept, (const OffT *)w.out_beg.p, incnt,
                                (WalkEntry<OffT> *)w.tmp_out.p, (OffT *)ctx->side_beg.p,
-                               (WalkEntry<OffT> *)ctx->side_ent.p);
+                               (WalkEntry<OffT> *)ctx->side_ent.p, (int32_t *)w.dv.p);
     if (rc)
         return rc;
     ctx->side_entries = x_in + x_out;
@@ -780,7 +868,7 @@ int tile_walk_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
         return rc;
     int *kept = (int *)ctx->side_cnt.p, *incnt = kept + rows;
     rc = wi_count<OffT, true>(ctx, beg, adj, rows, cols, degR, degC, kept, (OffT *)tw.out_beg.p,
-                              nullptr);
+                              nullptr, nullptr);
     if (rc)
         return rc;
     WalkIndexState &w = ctx->wi;
@@ -791,7 +879,7 @@ int tile_walk_build(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
         return rc;
     return wi_build<OffT, true>(ctx, beg, adj, rows, cols, degR, degC, kept,
                                 (const OffT *)tw.out_beg.p, incnt, (int32_t *)tw.out_ids.p,
-                                (OffT *)tw.in_beg.p, (int32_t *)tw.in_ids.p);
+                                (OffT *)tw.in_beg.p, (int32_t *)tw.in_ids.p, nullptr);
 }
 template int tile_walk_build<int32_t>(Ctx *, const int32_t *, const int32_t *, long long,
                                       long long, const int32_t *, const int32_t *, TileWalk &);
