@@ -774,16 +774,32 @@ tc_count_kernel(TcArgs<OffT> a)
         // list order is kept across shards.  A wave starts on shard
         // blockIdx % kQueueShards and moves on when a shard runs dry; after
         // finding every shard dry once it is done.
-        int shard = (int)(blockIdx.x % kQueueShards);
+        //
+        // Batches instead give shard s the s-th contiguous eighth of the
+        // rank's chunks and start a wave on the shard of its XCD: the list
+        // is in graph order, so each XCD counts its own graphs and their
+        // lists stay in its L2 (10k-graph ER/BA batch: 4.8% faster).  A
+        // single large graph keeps the interleave, which spreads each
+        // vertex range over all eight L2s (R-MAT 26: contiguous shards
+        // measured up to 3% slower).
+        constexpr bool kContig = kMode == kModeBatch;
+        int shard = kContig ? xcc_id() : (int)(blockIdx.x % kQueueShards);
+        long long per = 0;  // chunks per contiguous shard
+        if constexpr (kContig) {
+            const long long nchunk = (le - lb + a.chunk_wave - 1) / a.chunk_wave;
+            const long long nmine = nchunk > a.rank ? (nchunk - a.rank + a.nranks - 1) / a.nranks : 0;
+            per = (nmine + kQueueShards - 1) / kQueueShards;
+        }
         int dry = 0;
         while (dry < kQueueShards) {
             unsigned int k = 0;
             if (lane == 0)
                 k = atomicAdd(&a.ctl->shard[shard * kShardStride], 1u);
             k = __builtin_amdgcn_readfirstlane(k);
-            const long long j = (long long)k * kQueueShards + shard;
+            const long long j = kContig ? (long long)shard * per + (long long)k
+                                        : (long long)k * kQueueShards + shard;
             const long long first = lb + (j * a.nranks + a.rank) * a.chunk_wave;
-            if (first >= le) {
+            if ((kContig && (long long)k >= per) || first >= le) {
                 dry++;
                 shard = shard + 1 == kQueueShards ? 0 : shard + 1;
                 continue;

@@ -298,10 +298,6 @@ wi_rank_kernel(const WiSlice *__restrict__ sl, const Rec *__restrict__ recs,
 // own XCD's queue first, then the others', so one XCD's L2 holds the lines
 // a bin's concurrent chunks fill.  The register only steers locality: any
 // workgroup may take any chunk.
-__device__ __forceinline__ int xcc_id()
-{
-    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1));  // HW_REG_XCC_ID
-}
 
 template <typename OffT, bool kTile>
 struct PlaceStage {

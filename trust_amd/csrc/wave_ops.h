@@ -72,4 +72,13 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
     return x;
 }
 
+// The XCD (accelerator complex die, 0..7 on MI355X) this wave runs on, from
+// the hardware register XCC_ID.  Each XCD has its own L2, so work that
+// shares data is steered to one XCD; callers use it only for locality, never
+// for correctness.
+__device__ __forceinline__ int xcc_id()
+{
+    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7);  // HW_REG_XCC_ID
+}
+
 }  // namespace trust
