@@ -279,15 +279,21 @@ def test_segments_offset_lookup_flag(sctx, batch):
 
 
 def test_segments_beside_learned_schedule(sctx, batch):
-    # the learned plan is built over the whole int64 batch: its counts run the
-    # int64 kernel between segmented degree counts, eagerly and captured
+    # each int32 segment gets its own plan, learned or degree split, and the
+    # two schedules alternate, eagerly and captured
     beg, adj, vb, want = batch
     assert sctx.batch_segments(int(np.diff(beg[vb]).max())) > 1
     sctx.train_cost_model(epochs=3, max_samples=20000, seed=2)
     for kw in (dict(schedule=T.SCHED_LEARNED), dict(schedule=T.SCHED_DEGREE),
                dict(schedule=T.SCHED_LEARNED, flags=T.FLAG_HIP_GRAPH),
                dict(schedule=T.SCHED_DEGREE, flags=T.FLAG_HIP_GRAPH),
-               dict(schedule=T.SCHED_LEARNED, flags=T.FLAG_HIP_GRAPH)):
+               dict(schedule=T.SCHED_LEARNED, flags=T.FLAG_HIP_GRAPH),
+               dict(schedule=T.SCHED_LEARNED, threads=1024),
+               dict(schedule=T.SCHED_LEARNED, flags=T.FLAG_FORCE_SPILL)):
         counts, _, r = sctx.count_batch(**kw)
         assert np.array_equal(counts, want), kw
         assert r.schedule_used == kw["schedule"]
+        # every segment's plan lists each of its vertices with d+ >= 2 once
+        assert r.n_spill + r.n_hub + r.n_wave == int(np.sum(np.diff(beg) >= 2)), kw
+    r = sctx.count(schedule=T.SCHED_LEARNED)
+    assert r.triangles == int(want.sum()) and r.schedule_used == T.SCHED_LEARNED

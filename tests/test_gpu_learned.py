@@ -180,6 +180,31 @@ def test_judge_and_auto_schedule(gpu, rmat16):
         assert c.count(schedule=T.SCHED_AUTO).schedule_used == T.SCHED_LEARNED
 
 
+def test_learned_features_are_a_kept_index(gpu, rmat16):
+    # the learned plan's features depend on the graph and the walk alone: the
+    # first learned count of a graph makes them (index time), later ones reuse
+    # them; TRUST_FLAG_FRESH_INDEX and a new load drop them, and each walk
+    # has its own
+    beg, adj, want = rmat16
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        c.train_cost_model(epochs=2, max_samples=5000, seed=4)
+        for walk in (T.FLAG_REFERENCE_WALK, T.FLAG_SHORT_WALK):
+            kw = dict(schedule=T.SCHED_LEARNED, flags=walk)
+            first = c.count(**dict(kw, flags=walk | T.FLAG_FRESH_INDEX))
+            again = c.count(**kw)
+            assert first.triangles == again.triangles == want
+            assert first.schedule_used == again.schedule_used == T.SCHED_LEARNED
+            assert first.index_seconds > 0
+            assert again.index_seconds == 0, walk
+            assert again.seconds == pytest.approx(again.plan_seconds + again.count_seconds)
+        # the other walk's features were made last: the reference walk remakes its own
+        assert c.count(schedule=T.SCHED_LEARNED, flags=T.FLAG_REFERENCE_WALK).index_seconds > 0
+        c.load(beg, adj)
+        r = c.count(schedule=T.SCHED_LEARNED, flags=T.FLAG_REFERENCE_WALK)
+        assert r.triangles == want and r.index_seconds > 0
+
+
 def test_learned_order_counts_exactly_rmat20(gpu):
     # the trained network's order on a mid-size R-MAT counts exactly what the
     # degree split counts (times are printed, not asserted)

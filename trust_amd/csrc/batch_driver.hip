@@ -166,9 +166,13 @@ int enqueue_segments(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
             sv.ngraphs = (int)(q.g1 - q.g0);
             sv.graph_counts = out + q.g0;
             sv.graph_done = out + ng + q.g0;
-            rc = enqueue_count<int32_t>(ctx, v, q.v1 - q.v0, true, p, R.L, ctl + k, mode, nullptr,
-                                        nullptr, pass == 1, ctx->stream, nullptr, nullptr, &sv,
-                                        pass == 0);
+            if (pass == 0 && R.learned)
+                rc = enqueue_learned_plan_segment(ctx, p, R.L, ctl + k, ctx->stream, v, q.v1 - q.v0,
+                                                  q.v0);
+            else
+                rc = enqueue_count<int32_t>(ctx, v, q.v1 - q.v0, true, p, R.L, ctl + k, mode,
+                                            nullptr, nullptr, pass == 1, ctx->stream, nullptr,
+                                            nullptr, &sv, pass == 0);
             if (rc)
                 return rc;
         }

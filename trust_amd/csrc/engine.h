@@ -166,6 +166,10 @@ struct Ctx {
     // learned scheduler state
     DevBuf feat, tiers, cost, mlp_params;
     DevBuf walk_acc;  // per-vertex sum and max of the short-side walk's list lengths
+    // the walk whose per-vertex features feat holds (0: N+ rows, 1: the
+    // short-side walk's rows; -1: none): they depend on the graph alone, so
+    // a context keeps them like the walk indexes
+    int feat_walk = -1;
     // start index of the reference walk (single graphs, int32 offsets): per
     // entry of adj, its list start and length
     DevBuf ref_ent;
@@ -271,13 +275,22 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
 int finish_resident(Ctx *ctx, ResidentRun &R, const TcCtl &h, double t_plan, double t_count,
                     trust_result *r);
 
-// Learned scheduler (learned.hip): enqueues features, MLP inference, tier
-// choice and the longest-first counting sort that fill ctx->list and
-// ctl->list_begin for the resident graph.
+// Learned scheduler (learned.hip): enqueues MLP inference over the features
+// ensure_features made, tier choice and the longest-first counting sort that
+// fill ctx->list and ctl->list_begin for the resident graph.
 // begWalk: the walk offsets when the count takes the short-side walk, else
 // null (the walk is N+(u)).
 int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
                          hipStream_t s, const void *begWalk);
+// Makes ctx->feat hold the learned plan's per-vertex features of the given
+// walk (computed once per graph and walk; the caller times it as index work).
+int ensure_features(Ctx *ctx, bool walk);
+// The same plan for one int32 segment of a batch (views as the segment's
+// counting launch reads them; V its vertices, v0 its first): features, cost
+// and list go to the segment's slices of the context's buffers, the tier
+// bounds to ctl.
+int enqueue_learned_plan_segment(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
+                                 hipStream_t s, const CsrViews &v, int64_t V, int64_t v0);
 // Instrumented wave-only pass, GPU features, GPU training of the MLP.
 int train_cost_model(Ctx *ctx, const trust_params &p, int epochs, long long max_samples,
                      uint64_t seed, double *loss_out, double *seconds_out);

@@ -549,6 +549,7 @@ int ensure_walk_index(Ctx *ctx)
 // loaded graph without reloading it).
 static void drop_indexes(Ctx *ctx)
 {
+    ctx->feat_walk = -1;
     ctx->side_stats_ready = ctx->side_built = ctx->side_rows_built = false;
     ctx->wi.counted = false;
     ctx->side_entries = -1;
@@ -593,11 +594,11 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
     // found the degree split faster on this graph), else the degree split
     R.learned = ctx->mlp_ready && (p.schedule == TRUST_SCHED_LEARNED ||
                                    (p.schedule == TRUST_SCHED_AUTO && ctx->judge != TRUST_SCHED_DEGREE));
-    // An int64 batch with segments runs the int32 kernel per segment; the
-    // short-side walk and the learned plan are built over the whole batch
-    // and keep the int64 kernel, and so does the instrumented mode, whose
-    // outputs are indexed by vertex.
-    R.segmented = !ctx->segs.empty() && mode != kModeInstr && !R.side && !R.learned;
+    // An int64 batch with segments runs the int32 kernel per segment, each
+    // with its own plan (degree split or learned); the short-side walk is
+    // built over the whole batch and keeps the int64 kernel, and so does the
+    // instrumented mode, whose outputs are indexed by vertex.
+    R.segmented = !ctx->segs.empty() && mode != kModeInstr && !R.side;
     rc = resolve_launch(ctx, p, g.max_deg, g.V, g.off64 && !R.segmented, R.L, mode);
     if (rc)
         return rc;
@@ -618,6 +619,12 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
     if (R.side && R.learned) {
         int64_t n;
         if ((rc = side_entry_count(ctx, &n)) || (rc = ensure_walk_rows(ctx)))
+            return rc;
+    }
+    // the learned plan's per-vertex features depend on the graph and the
+    // walk alone: made once, as index work, and kept for later plans
+    if (R.learned && ctx->feat_walk != (R.side ? 1 : 0)) {
+        if ((rc = index_begin(ctx, R)) || (rc = ensure_features(ctx, R.side)))
             return rc;
     }
     if (R.index_ev)

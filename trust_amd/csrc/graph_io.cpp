@@ -104,6 +104,28 @@ int64_t host_max_degree(const int64_t *beg, int64_t V)
     return maxd;
 }
 
+void host_parallel_copy(void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t kPiece = 4u << 20;
+    const long long pieces = (long long)((bytes + kPiece - 1) / kPiece);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < pieces; i++) {
+        const size_t o = (size_t)i * kPiece;
+        memcpy((char *)dst + o, (const char *)src + o, std::min(kPiece, bytes - o));
+    }
+}
+
+void host_tile_row_degrees(const trust_csr *tiles, int n, int c, int64_t rows, int32_t *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < rows; r++) {
+        int64_t x = 0;
+        for (int b = 0; b < n; b++)
+            x += tiles[(size_t)c * n + b].beg[r + 1] - tiles[(size_t)c * n + b].beg[r];
+        out[r] = (int32_t)std::min<int64_t>(x, INT32_MAX);
+    }
+}
+
 int export_csr(HostCsr &&src, trust_csr *out)
 {
     out->vertices = src.V;

@@ -35,6 +35,13 @@ int load_folder(const std::string &folder, HostCsr &out);
 // Largest beg[u+1]-beg[u] (OpenMP, host).
 int64_t host_max_degree(const int64_t *beg, int64_t V);
 
+// memcpy with all OpenMP threads, in 4 MB pieces.
+void host_parallel_copy(void *dst, const void *src, size_t bytes);
+
+// Row sums of tile class c of an n x n partition (tiles[c*n + b], b < n):
+// the global out-degree of each of its `rows` vertices, saturated to int32.
+void host_tile_row_degrees(const trust_csr *tiles, int n, int c, int64_t rows, int32_t *out);
+
 // Parallel LSD radix sort of 64-bit keys (stable); tmp must hold n keys.
 void radix_sort_u64(uint64_t *keys, uint64_t *tmp, size_t n, int key_bits);
 

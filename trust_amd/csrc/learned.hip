@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) features_kernel(const OffT *__restrict__ 
         wave_sync();
         if (lane < n) {
             const long long u = u0 + lane;
-            const float dd = (float)(begD[u + 1] - begD[u]);  // table size d+(u)
+            const float dd = (float)d;  // table size d+(u)
             feat[3 * u] = log2f(1.f + dd) * kFeatScale;
             feat[3 * u + 1] = log2f(1.f + (float)lsum[lane]) * kFeatScale;
             feat[3 * u + 2] = log2f(1.f + (float)lmax[lane]) * kFeatScale;
@@ -466,44 +466,51 @@ static int enqueue_features(Ctx *ctx, float *feat, hipStream_t s, bool walk)
     return TRUST_OK;
 }
 
-static int features_dispatch(Ctx *ctx, float *feat, hipStream_t s, bool walk = false)
+// Features of the given walk into ctx->feat (which then holds that walk's).
+static int features_dispatch(Ctx *ctx, hipStream_t s, bool walk = false)
 {
-    if (walk) {
-        int rc = ensure_walk_rows(ctx);  // written before any capture (prepare_resident)
-        if (rc)
-            return rc;
-    }
-    return ctx->g.off64 ? enqueue_features<int64_t>(ctx, feat, s, walk)
-                        : enqueue_features<int32_t>(ctx, feat, s, walk);
-}
-
-template <typename OffT>
-static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
-                                  TcCtl *ctl, hipStream_t s, const void *begWalk)
-{
-    const long long V = ctx->g.V;
-    int rc;
-    if ((rc = ctx->feat.ensure(sizeof(float) * 3 * std::max<long long>(V, 1))) ||
-        (rc = ctx->cost.ensure(sizeof(float) * std::max<long long>(V, 1))) ||
-        (rc = ctx->tiers.ensure(sizeof(unsigned short) * std::max<long long>(V, 1))) ||
-        (rc = ctx->list.ensure(sizeof(int32_t) * std::max<long long>(V, 1))))
-        return rc;
-    HIPCHK(launch_fill_u32((uint32_t *)ctl, 0u, sizeof(TcCtl) / 4, s));
-    rc = enqueue_features<OffT>(ctx, (float *)ctx->feat.p, s, begWalk != nullptr);
+    int rc = ctx->feat.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(ctx->g.V, 1));
     if (rc)
         return rc;
+    ctx->feat_walk = -1;
+    if (walk && (rc = ensure_walk_rows(ctx)))  // written before any capture (prepare_resident)
+        return rc;
+    float *feat = (float *)ctx->feat.p;
+    rc = ctx->g.off64 ? enqueue_features<int64_t>(ctx, feat, s, walk)
+                      : enqueue_features<int32_t>(ctx, feat, s, walk);
+    if (rc)
+        return rc;
+    ctx->feat_walk = walk ? 1 : 0;
+    return TRUST_OK;
+}
+
+int ensure_features(Ctx *ctx, bool walk)
+{
+    if (ctx->feat_walk == (walk ? 1 : 0))
+        return TRUST_OK;
+    return features_dispatch(ctx, ctx->stream, walk);
+}
+
+// Cost prediction, tier keys and the longest-first list of the vertices
+// [0, V) of one hash view (beg): list and ctl->list_begin.  feat, cost and
+// keys are this range's slices.
+template <typename OffT>
+static int plan_from_features(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
+                              hipStream_t s, const OffT *beg, const OffT *begWalk, long long V,
+                              const float *feat, float *cost, unsigned short *keys, int32_t *list)
+{
     const int cb = (int)std::max<long long>(1, std::min<long long>((V + 255) / 256, 4096));
     // block sums (cb doubles) + total (1 double) live in block_counts' space
     const long long per_block = std::max<long long>(4096, (V + 1023) / 1024);
     const int nb = (int)std::max<long long>(1, (V + per_block - 1) / per_block);
-    if ((rc = ctx->block_counts.ensure(sizeof(long long) * ((size_t)kKeys * nb + cb + 1))))
+    int rc = ctx->block_counts.ensure(sizeof(long long) * ((size_t)kKeys * nb + cb + 1));
+    if (rc)
         return rc;
     long long *counts = (long long *)ctx->block_counts.p;
     double *bsum = (double *)(counts + (size_t)kKeys * nb);
     double *total = bsum + cb;
-    hipLaunchKernelGGL(cost_kernel<OffT>, dim3(cb), dim3(256), 0, s, (const OffT *)ctx->g.beg.p, V,
-                       (const float *)ctx->feat.p, (const float *)ctx->mlp_params.p,
-                       (float *)ctx->cost.p, bsum);
+    hipLaunchKernelGGL(cost_kernel<OffT>, dim3(cb), dim3(256), 0, s, beg, V, feat,
+                       (const float *)ctx->mlp_params.p, cost, bsum);
     hipLaunchKernelGGL(sum_blocks_kernel, dim3(1), dim3(64), 0, s, bsum, cb, total);
     KeyArgs k;
     k.V = V;
@@ -516,19 +523,65 @@ static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchP
     k.hub_threshold = p.hub_threshold;
     k.split_fraction = 0.5;
     k.waves = (double)L.main.blocks * (L.main.threads / kWave);
-    hipLaunchKernelGGL(keys_kernel<OffT>, dim3(cb), dim3(256), 0, s, (const OffT *)ctx->g.beg.p,
-                       (const OffT *)begWalk, k, (const float *)ctx->cost.p, (const double *)total,
-                       (unsigned short *)ctx->tiers.p);
-    hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, s,
-                       (const unsigned short *)ctx->tiers.p, V, per_block, counts, nb);
+    hipLaunchKernelGGL(keys_kernel<OffT>, dim3(cb), dim3(256), 0, s, beg, begWalk, k,
+                       (const float *)cost, (const double *)total, keys);
+    hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, s, (const unsigned short *)keys, V,
+                       per_block, counts, nb);
     // a batch drains its graphs in 32 vertex ranges (progressive completion)
     const int nsb = ctx->batch_ngraphs > 1 ? 32 : 1;
     hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, s, counts, nb, nsb, ctl);
-    hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, s,
-                       (const unsigned short *)ctx->tiers.p, V, per_block, (const long long *)counts,
-                       nb, (int32_t *)ctx->list.p);
+    hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, s, (const unsigned short *)keys,
+                       V, per_block, (const long long *)counts, nb, list);
     HIPCHK(hipGetLastError());
     return TRUST_OK;
+}
+
+static int ensure_plan_buffers(Ctx *ctx)
+{
+    const long long V = std::max<long long>(ctx->g.V, 1);
+    int rc;
+    if ((rc = ctx->feat.ensure(sizeof(float) * 3 * V)) ||
+        (rc = ctx->cost.ensure(sizeof(float) * V)) ||
+        (rc = ctx->tiers.ensure(sizeof(unsigned short) * V)) ||
+        (rc = ctx->list.ensure(sizeof(int32_t) * V)))
+        return rc;
+    return TRUST_OK;
+}
+
+template <typename OffT>
+static int enqueue_learned_plan_t(Ctx *ctx, const trust_params &p, const LaunchPlan &L,
+                                  TcCtl *ctl, hipStream_t s, const void *begWalk)
+{
+    if (ctx->feat_walk != (begWalk != nullptr ? 1 : 0))
+        return fail(TRUST_ERR_ARG, "learned plan: features of this walk not prepared");
+    int rc = ensure_plan_buffers(ctx);
+    if (rc)
+        return rc;
+    HIPCHK(launch_fill_u32((uint32_t *)ctl, 0u, sizeof(TcCtl) / 4, s));
+    return plan_from_features<OffT>(ctx, p, L, ctl, s, (const OffT *)ctx->g.beg.p,
+                                    (const OffT *)begWalk, ctx->g.V, (const float *)ctx->feat.p,
+                                    (float *)ctx->cost.p, (unsigned short *)ctx->tiers.p,
+                                    (int32_t *)ctx->list.p);
+}
+
+int enqueue_learned_plan_segment(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
+                                 hipStream_t s, const CsrViews &v, int64_t V, int64_t v0)
+{
+    if (!ctx->mlp_ready)
+        return fail(TRUST_ERR_ARG, "no cost model");
+    if (ctx->feat_walk != 0)  // segments walk N+: the features of the whole batch, by vertex
+        return fail(TRUST_ERR_ARG, "learned plan: features not prepared");
+    int rc = ensure_plan_buffers(ctx);
+    if (rc)
+        return rc;
+    HIPCHK(launch_fill_u32((uint32_t *)ctl, 0u, sizeof(TcCtl) / 4, s));
+    if (V == 0)
+        return TRUST_OK;
+    const float *feat = (const float *)ctx->feat.p + 3 * v0;
+    return plan_from_features<int32_t>(ctx, p, L, ctl, s, (const int32_t *)v.begH, nullptr, V, feat,
+                                       (float *)ctx->cost.p + v0,
+                                       (unsigned short *)ctx->tiers.p + v0,
+                                       (int32_t *)ctx->list.p + v0);
 }
 
 int enqueue_learned_plan(Ctx *ctx, const trust_params &p, const LaunchPlan &L, TcCtl *ctl,
@@ -562,7 +615,7 @@ int train_cost_model(Ctx *ctx, const trust_params &p_in, int epochs, long long m
     if (rc)
         return rc;
     // features of the walk the instrumented count took
-    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream, r.walk_used != 0);
+    rc = features_dispatch(ctx, ctx->stream, r.walk_used != 0);
     if (rc)
         return rc;
     std::vector<float> feat((size_t)3 * V);
@@ -660,7 +713,7 @@ extern "C" int trust_ctx_features(trust_ctx *c, float *feat)
     int rc = ctx->feat.ensure(sizeof(float) * 3 * std::max<long long>(V, 1));
     if (rc)
         return rc;
-    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream);
+    rc = features_dispatch(ctx, ctx->stream);
     if (rc)
         return rc;
     HIPCHK(hipMemcpyAsync(feat, ctx->feat.p, sizeof(float) * 3 * V, hipMemcpyDeviceToHost,
@@ -681,7 +734,7 @@ extern "C" int trust_ctx_walk_features(trust_ctx *c, float *feat)
         return rc;
     if ((rc = ensure_walk_index(ctx)))
         return rc;
-    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream, ctx->g.V > 0 && ctx->g.E > 0);
+    rc = features_dispatch(ctx, ctx->stream, ctx->g.V > 0 && ctx->g.E > 0);
     if (rc)
         return rc;
     HIPCHK(hipMemcpyAsync(feat, ctx->feat.p, sizeof(float) * 3 * V, hipMemcpyDeviceToHost,
@@ -763,7 +816,7 @@ extern "C" int trust_ctx_predict_costs(trust_ctx *c, float *cost)
         (rc = ctx->cost.ensure(sizeof(float) * std::max<long long>(V, 1))) ||
         (rc = ctx->block_counts.ensure(sizeof(double) * (cb + 1))))
         return rc;
-    rc = features_dispatch(ctx, (float *)ctx->feat.p, ctx->stream);
+    rc = features_dispatch(ctx, ctx->stream);
     if (rc)
         return rc;
     if (ctx->g.off64)

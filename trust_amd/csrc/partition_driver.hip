@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "internal.h"
 
 namespace trust {
 
@@ -56,6 +57,73 @@ static int check_job_tiles(const trust_csr &ht, const trust_csr &walk, const tru
 }
 
 namespace {
+
+// Two pinned host buffers that tile uploads pass through.  The host copies a
+// tile into one with all its threads while the DMA engine drains the other,
+// so an upload costs the caller a parallel memcpy and the transfer itself
+// runs asynchronously.  A pageable hipMemcpyAsync instead stages every byte
+// through the runtime's own buffers on the calling thread before it returns.
+// Allocated with hipHostMalloc.
+struct PinnedStage {
+    char *buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};  // the DMA out of buf[b] has finished
+    int next = 0;
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage &) = delete;
+    PinnedStage &operator=(const PinnedStage &) = delete;
+    ~PinnedStage()
+    {
+        for (int b = 0; b < 2; b++) {
+            if (done[b]) {
+                (void)hipEventSynchronize(done[b]);
+                (void)hipEventDestroy(done[b]);
+            }
+            if (buf[b])
+                (void)hipHostFree(buf[b]);
+        }
+    }
+    // false when pinned memory is not available (uploads then stay pageable)
+    bool init(size_t n)
+    {
+        for (int b = 0; b < 2; b++)
+            if (hipHostMalloc((void **)&buf[b], n, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&done[b], hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                return false;
+            }
+        return true;
+    }
+};
+
+// Tile t to the device CSR g through the next pinned buffer, on stream s.
+static int upload_pinned(Ctx *ctx, PinnedStage &st, DevCsr &g, const trust_csr &t, bool off64,
+                         hipStream_t s)
+{
+    const size_t ab = sizeof(int32_t) * (size_t)t.edges;
+    const size_t boff = (ab + 255) & ~(size_t)255;
+    const size_t bb = sizeof(int64_t) * (size_t)(t.vertices + 1);
+    const int b = st.next;
+    st.next ^= 1;
+    HIPCHK(hipEventSynchronize(st.done[b]));
+    char *h = st.buf[b];
+    host_parallel_copy(h, t.adj, ab);
+    host_parallel_copy(h + boff, t.beg, bb);
+    g.V = t.vertices;
+    g.E = t.edges;
+    g.off64 = off64;
+    g.max_deg = host_max_degree(t.beg, t.vertices);
+    if (ab > 0)
+        HIPCHK(hipMemcpyAsync(g.adj.p, h, ab, hipMemcpyHostToDevice, s));
+    if (off64) {
+        HIPCHK(hipMemcpyAsync(g.beg.p, h + boff, bb, hipMemcpyHostToDevice, s));
+    } else {
+        HIPCHK(hipMemcpyAsync(ctx->staging.p, h + boff, bb, hipMemcpyHostToDevice, s));
+        HIPCHK(launch_offsets_to_i32((const int64_t *)ctx->staging.p, (int32_t *)g.beg.p,
+                                     t.vertices + 1, s));
+    }
+    HIPCHK(hipEventRecord(st.done[b], s));
+    return TRUST_OK;
+}
 
 struct TileSlot {
     DevCsr g;
@@ -120,6 +188,11 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         if (rc)
             return rc;
     }
+    // uploads go through two pinned buffers, each sized for the largest tile
+    PinnedStage stage;
+    const size_t stage_bytes = ((sizeof(int32_t) * (size_t)maxE + 255) & ~(size_t)255) +
+                               sizeof(int64_t) * (size_t)(maxV + 1);
+    const bool pinned_uploads = stage.init(stage_bytes);
     rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(maxV, 1));
     if (!rc)
         rc = ctx->block_counts.ensure(sizeof(long long) * kNumTiers *
@@ -159,16 +232,8 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
     if (short_walk) {
         // global out-degree of every vertex, class by class (sum over its tile row)
         std::vector<int32_t> d((size_t)deg_off[n]);
-        for (int c = 0; c < n; c++) {
-            int32_t *dc = d.data() + deg_off[c];
-#pragma omp parallel for schedule(static)
-            for (int64_t r = 0; r < rows_of[c]; r++) {
-                int64_t x = 0;
-                for (int b = 0; b < n; b++)
-                    x += tiles[(size_t)c * n + b].beg[r + 1] - tiles[(size_t)c * n + b].beg[r];
-                dc[r] = (int32_t)std::min<int64_t>(x, INT32_MAX);
-            }
-        }
+        for (int c = 0; c < n; c++)
+            host_tile_row_degrees(tiles, n, c, rows_of[c], d.data() + deg_off[c]);
         if ((rc = degs.ensure(sizeof(int32_t) * std::max<size_t>(d.size(), 1))))
             return rc;
         HIPCHK(hipMemcpy(degs.p, d.data(), sizeof(int32_t) * d.size(), hipMemcpyHostToDevice));
@@ -181,8 +246,12 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
 
     // per job: events around its counting launches (the count's share of the
     // streamed time); host time inside the tile uploads; the tile walks' builds
-    double copy_host_s = 0, build_s = 0;
-    std::vector<hipEvent_t> jev((size_t)(2 * njobs), nullptr);
+    double copy_host_s = 0;
+    // and events around each tile walk's build (at most one per tile (i,k)):
+    // the build's time on the stream, host readbacks between its passes included
+    std::vector<hipEvent_t> jev((size_t)(2 * njobs + 2 * n * n), nullptr);
+    hipEvent_t *wev = jev.data() + 2 * njobs;
+    int nbuilds = 0;
     struct EventGuard {
         std::vector<hipEvent_t> &e;
         ~EventGuard()
@@ -224,11 +293,14 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
             HIPCHK(hipStreamWaitEvent(ctx->copy_stream, v.released, 0));
         const trust_csr &src = tiles[t];
         const auto c0 = std::chrono::steady_clock::now();
-        int rc2 = upload_csr(ctx, v.g, src.beg, src.adj, src.vertices, src.edges, off64,
-                             ctx->copy_stream);
+        int rc2 = pinned_uploads
+                      ? upload_pinned(ctx, stage, v.g, src, off64, ctx->copy_stream)
+                      : upload_csr(ctx, v.g, src.beg, src.adj, src.vertices, src.edges, off64,
+                                   ctx->copy_stream);
         if (rc2)
             return rc2;
-        // pageable host memory: the call returns once the runtime has staged the bytes
+        // host time of the upload: the wait for a free pinned buffer and the
+        // parallel memcpy into it (pageable: the runtime staging every byte)
         copy_host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
         HIPCHK(hipEventRecord(v.ready, ctx->copy_stream));
         v.tile = t;
@@ -262,7 +334,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
             HIPCHK(hipStreamWaitEvent(ctx->stream, slots[ws].ready, 0));
             const DevCsr &wg = slots[ws].g;
             const int32_t *dg = (const int32_t *)degs.p;
-            const auto b0 = std::chrono::steady_clock::now();
+            HIPCHK(hipEventRecord(wev[2 * nbuilds], ctx->stream));
             rc = off64 ? tile_walk_build<int64_t>(ctx, (const int64_t *)wg.beg.p,
                                                   (const int32_t *)wg.adj.p, rows_of[ji],
                                                   rows_of[jk], dg + deg_off[ji], dg + deg_off[jk],
@@ -273,7 +345,8 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                                                   walks[(size_t)jk]);
             if (rc)
                 return rc;
-            build_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - b0).count();
+            HIPCHK(hipEventRecord(wev[2 * nbuilds + 1], ctx->stream));
+            nbuilds++;
             HIPCHK(hipEventRecord(slots[ws].released, ctx->stream));
             walk_row[(size_t)jk] = ji;
         }
@@ -353,6 +426,12 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         HIPCHK(hipEventElapsedTime(&jm, jev[(size_t)(2 * j)], jev[(size_t)(2 * j + 1)]));
         count_s += jm * 1e-3;
     }
+    double build_s = 0;
+    for (int b = 0; b < nbuilds; b++) {
+        float bm = 0;
+        HIPCHK(hipEventElapsedTime(&bm, wev[2 * b], wev[2 * b + 1]));
+        build_s += bm * 1e-3;
+    }
     uint64_t total = 0, probes = 0, rebuilds = 0;
     for (long long j = 0; j < njobs; j++) {
         const TcCtl &x = h[(size_t)(short_walk ? 2 * j : j)];
@@ -371,7 +450,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         r->seconds = ms * 1e-3;       // the whole stream: copies, walk builds, counts
         r->count_seconds = count_s;   // the jobs' counting launches alone
         r->index_seconds = build_s;   // the tile walks' builds
-        r->h2d_seconds = copy_host_s; // host time inside the pageable tile uploads
+        r->h2d_seconds = copy_host_s;  // host time inside the tile uploads
         r->gteps = r->seconds > 0 ? (double)Esum / r->seconds / 1e9 : 0;
         r->alg_bytes = alg_bytes(Vsum, Esum, probes, off64);
         r->threads = L.main.threads;
