@@ -150,6 +150,17 @@ typedef struct trust_ctx trust_ctx;
  * on any CUDA error instead (HRR, Without-graph-partition/herror.h:1-8). */
 const char *trust_last_error(void);
 const char *trust_version(void);
+/* Caps the device memory all of the library's buffers may hold together (all
+ * contexts and devices of the process; 0 = no cap, the default).  An
+ * allocation beyond the cap fails with TRUST_ERR_OOM exactly as a failed
+ * hipMalloc does, so the out-of-memory paths (the walk index's fallback to
+ * its low-scratch build, clean errors, a context that stays usable) can be
+ * exercised, and a caller can bound the library's footprint.  No counterpart
+ * in the reference, which exits on any allocation failure
+ * (Without-graph-partition/herror.h:1-8). */
+int trust_set_device_memory_cap(uint64_t bytes);
+/* Device bytes the library's buffers hold now (graphs, indexes, scratch). */
+uint64_t trust_device_memory_held(void);
 /* The launch fields stand for the reference's argv[3..5] and the MPI rank /
  * argv[2] (Without-graph-partition/mpi_main.cpp:78-81).
  * Defaults: threads 0, blocks 0, chunk 0, rank 0, total_process 1,
@@ -213,6 +224,22 @@ int trust_count_csr_noalloc(const int64_t *beg_dev, const int32_t *adj_dev, int6
  * tile (i,j) in the layout of Preprocess/partition.cpp:72-121.
  */
 int trust_count_tiles(const trust_csr *tiles, int n, const trust_params *p, trust_result *r);
+/* trust_count_tiles with a job journal, so that an interrupted stream resumes
+ * where it stopped.  The journal (a text file, created when missing) names
+ * the partition on its first line and gets one line per finished job,
+ * "job <id> <i> <j> <k> <triangles> <probes>", appended and flushed as each
+ * job's count reaches the host; jobs already in it are not counted again (the
+ * jobs' counts are additive).  A last line cut short is dropped; a journal of
+ * another partition is TRUST_ERR_FORMAT.  max_new_jobs > 0 stops after that
+ * many new jobs.  Returns the number of jobs still missing from the journal
+ * (0: r holds the whole count) or a negative error code; r sums every job in
+ * the journal, its times cover this call's work; job_counts (nullable, n^3)
+ * receives each job's count (0 for jobs still missing).  No counterpart in
+ * the reference, whose ranks each count one job and keep nothing
+ * (With-graph-partition/mpi_main.cpp). */
+int trust_count_tiles_journal(const trust_csr *tiles, int n, const trust_params *p,
+                              const char *journal, int64_t max_new_jobs, uint64_t *job_counts,
+                              trust_result *r);
 
 /* ---- resident graphs -------------------------------------------------- */
 /* No counterpart in the reference, whose every Triangle_count call loads,

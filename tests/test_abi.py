@@ -42,6 +42,16 @@ def test_version_and_defaults():
     assert p.total_process == 1 and p.hub_threshold == 100 and p.schedule == T.SCHED_DEGREE
 
 
+def test_device_memory_cap_setting():
+    # no device needed: the cap and the held-bytes counter are host state
+    assert T.device_memory_held() >= 0
+    try:
+        T.set_device_memory_cap(1 << 20)
+        T.set_device_memory_cap(0)
+    finally:
+        T.set_device_memory_cap(0)
+
+
 def _write(folder, beg, adj):
     os.makedirs(folder, exist_ok=True)
     np.asarray(beg, np.int64).tofile(os.path.join(folder, "begin.bin"))

@@ -70,3 +70,43 @@ def test_cli_partitioned(gpu, rmat14, tmp_path):
                          capture_output=True, text=True, check=True, timeout=120).stdout
     line = out.strip().splitlines()[-1]
     assert line.split(",")[1] == f"triangle:{want}"
+
+
+def test_journal_resume(gpu, rmat14, tmp_path):
+    # an interrupted stream resumes from its journal: stop after a few jobs
+    # (forced from the host), resume twice, and every job's count, the sum
+    # and the journal's lines must match the uninterrupted stream
+    beg, adj, want = rmat14
+    n = 3
+    tiles = T.partition_csr(beg, adj, n)
+    full = T.count_tiles(tiles, n)
+    assert full.triangles == want
+    path = str(tmp_path / "jobs.journal")
+    left, r1, _ = T.count_tiles_journal(tiles, n, path, max_new_jobs=5)
+    assert left == n ** 3 - 5 and r1.triangles <= want
+    lines = open(path).read().splitlines()
+    assert lines[0].startswith("trust-tiles-journal 1 n=3 ") and len(lines) == 6
+    left, r2, _ = T.count_tiles_journal(tiles, n, path, max_new_jobs=10)
+    assert left == n ** 3 - 15 and r1.triangles <= r2.triangles <= want
+    # a write cut short: the last line loses its newline and is counted again
+    text = open(path).read()
+    open(path, "w").write(text[:-3])
+    left, r3, jobs = T.count_tiles_journal(tiles, n, path)
+    assert left == 0 and r3.triangles == want and r3.probes == full.probes
+    lines = open(path).read().splitlines()
+    assert len(lines) == 1 + n ** 3
+    ids = sorted(int(x.split()[1]) for x in lines[1:])
+    assert ids == list(range(n ** 3))
+    for x in lines[1:]:
+        _, job, i, j, k, c, _w = x.split()
+        job, i, j, k, c = int(job), int(i), int(j), int(k), int(c)
+        assert (i, j, k) == (job // (n * n), job // n % n, job % n)
+        assert c == jobs[job]
+    assert int(jobs.sum()) == want
+    # a finished journal counts nothing more and gives the same result
+    left, r4, _ = T.count_tiles_journal(tiles, n, path)
+    assert left == 0 and r4.triangles == want and r4.count_seconds == 0
+    # a journal of another partition is refused
+    with pytest.raises(T.TrustError) as e:
+        T.count_tiles_journal(T.partition_csr(beg, adj, 2), 2, path)
+    assert e.value.code == T.ERR_FORMAT

@@ -91,6 +91,8 @@ _PF = C.POINTER(C.c_float)
 # name -> (restype, argtypes); every symbol of include/trust.h
 SIGNATURES = {
     "trust_last_error": (C.c_char_p, []),
+    "trust_set_device_memory_cap": (C.c_int, [C.c_uint64]),
+    "trust_device_memory_held": (C.c_uint64, []),
     "trust_version": (C.c_char_p, []),
     "trust_params_init": (None, [C.POINTER(Params)]),
     "trust_device_count": (C.c_int, []),
@@ -102,6 +104,8 @@ SIGNATURES = {
                                   C.POINTER(Result)]),
     "trust_count_tiles": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Params),
                                     C.POINTER(Result)]),
+    "trust_count_tiles_journal": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Params),
+                                            C.c_char_p, C.c_int64, _PU64, C.POINTER(Result)]),
     # device pointers (beg int64, adj int32), sizes, params, result, HIP stream
     "trust_count_csr_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
                                           C.POINTER(Params), C.POINTER(Result), C.c_void_p]),
@@ -208,6 +212,17 @@ def device_count() -> int:
     if n < 0:
         _check(n)
     return n
+
+
+def set_device_memory_cap(nbytes: int) -> None:
+    """Cap on the device bytes the library's buffers may hold together (0: none);
+    allocations beyond it fail with ERR_OOM."""
+    _check(lib().trust_set_device_memory_cap(nbytes))
+
+
+def device_memory_held() -> int:
+    """Device bytes the library's buffers hold now."""
+    return int(lib().trust_device_memory_held())
 
 
 def make_params(**kw) -> Params:
@@ -391,6 +406,25 @@ def count_tiles(tiles, n: int, **params) -> Result:
     r = Result()
     _check(lib().trust_count_tiles(arr, n, C.byref(p), C.byref(r)))
     return r
+
+
+def count_tiles_journal(tiles, n: int, journal: str, max_new_jobs: int = 0, **params):
+    """count_tiles that records each finished job in a journal file and skips
+    the jobs already there: (jobs still missing, Result, per-job counts)."""
+    arr = (Csr * (n * n))()
+    keep = []
+    for k, (b, a) in enumerate(tiles):
+        c, kk = _as_csr(b, a)
+        arr[k] = c
+        keep.append(kk)
+    p = make_params(**params)
+    r = Result()
+    jobs = np.zeros(n ** 3, np.uint64)
+    rc = lib().trust_count_tiles_journal(arr, n, C.byref(p), os.fsencode(journal), max_new_jobs,
+                                         jobs.ctypes.data_as(_PU64), C.byref(r))
+    if rc < 0:
+        _check(rc)
+    return rc, r, jobs
 
 
 def triangle_count(rank: int, folder: str, total_process: int = 1, threads: int = 0,

@@ -9,6 +9,8 @@
 // from hipEvents on the launch stream.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -26,6 +28,11 @@ int hip_fail(hipError_t e, const char *what)
     return fail(TRUST_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Device bytes held by the library's buffers (all contexts and devices of
+// the process) and the cap trust_set_device_memory_cap puts on them (0: none).
+static std::atomic<unsigned long long> g_dev_held{0};
+static std::atomic<unsigned long long> g_dev_cap{0};
+
 int DevBuf::ensure(size_t n)
 {
     if (n <= bytes)
@@ -33,8 +40,18 @@ int DevBuf::ensure(size_t n)
     release();
     if (n == 0)
         return TRUST_OK;
+    const unsigned long long held = g_dev_held.fetch_add(n) + n;
+    const unsigned long long cap = g_dev_cap.load();
+    if (cap && held > cap) {
+        g_dev_held.fetch_sub(n);
+        return fail(TRUST_ERR_OOM,
+                    "device allocation of %zu bytes would make the library hold %llu bytes, above "
+                    "its cap of %llu (trust_set_device_memory_cap)",
+                    n, held, cap);
+    }
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
+        g_dev_held.fetch_sub(n);
         p = nullptr;
         return hip_fail(e, "hipMalloc");
     }
@@ -44,8 +61,10 @@ int DevBuf::ensure(size_t n)
 
 void DevBuf::release()
 {
-    if (p)
+    if (p) {
         (void)hipFree(p);
+        g_dev_held.fetch_sub(bytes);
+    }
     p = nullptr;
     bytes = 0;
 }
@@ -400,16 +419,30 @@ static int index_begin(Ctx *ctx, ResidentRun &R)
 static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
 {
     DevCsr &g = ctx->g;
-    const bool binned = walk_index_binned(ctx);
+    // The binned build needs scratch beyond the index itself (per-edge d+,
+    // records, ranks); when HBM cannot hold it, the atomic build and its own
+    // statistics pass take over.
+    bool binned = walk_index_binned(ctx);
+    auto leave_binned = [&]() -> int {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        release_walk_scratch(ctx);
+        ctx->wi.counted = false;
+        binned = false;
+        return TRUST_OK;
+    };
     int rc;
     if (!force && !ctx->side_stats_ready) {
         if ((rc = index_begin(ctx, R)))
             return rc;
         if (binned) {
             // the binned build's count pass yields the probe totals too
-            if ((rc = walk_index_count(ctx)))
+            rc = walk_index_count(ctx);
+            if (rc == TRUST_ERR_OOM)
+                rc = leave_binned();
+            if (rc)
                 return rc;
-        } else {
+        }
+        if (!binned) {
             rc = ctx->side_stats.ensure(4 * sizeof(unsigned long long));
             if (rc)
                 return rc;
@@ -435,17 +468,15 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
         const size_t ob = g.off64 ? sizeof(int64_t) : sizeof(int32_t);
         bool done = false;
         if (binned) {
-            if (!ctx->wi.counted && (rc = walk_index_count(ctx)))
-                return rc;
-            rc = walk_index_build(ctx);
-            if (rc == TRUST_ERR_OOM) {  // not enough HBM for its scratch: the atomic build
-                HIPCHK(hipStreamSynchronize(ctx->stream));
-                release_walk_scratch(ctx);
-            } else if (rc) {
-                return rc;
-            } else {
+            rc = ctx->wi.counted ? TRUST_OK : walk_index_count(ctx);
+            if (!rc)
+                rc = walk_index_build(ctx);
+            if (rc == TRUST_ERR_OOM)
+                rc = leave_binned();
+            else if (!rc)
                 done = true;
-            }
+            if (rc)
+                return rc;
         }
         if (!done) {
             rc = ctx->side_cnt.ensure(2 * sizeof(int) * (size_t)g.V);
@@ -481,10 +512,14 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
     return TRUST_OK;
 }
 
-// Frees the binned build's scratch (records, ranks, bases, out-part copy).
+// Frees the binned build's scratch (per-edge d+, histograms, records,
+// ranks, bases, out-part copy).
 void release_walk_scratch(Ctx *ctx)
 {
     WalkIndexState &w = ctx->wi;
+    w.dv.release();
+    w.M.release();
+    w.O.release();
     w.recs.release();
     w.rel.release();
     w.sbase.release();
@@ -855,6 +890,15 @@ int load_ctx_csr(Ctx *ctx, const int64_t *beg, const int32_t *adj, int64_t V, in
 }  // namespace trust
 
 using namespace trust;
+
+extern "C" int trust_set_device_memory_cap(uint64_t bytes)
+{
+    clear_error();
+    g_dev_cap.store((unsigned long long)bytes);
+    return TRUST_OK;
+}
+
+extern "C" uint64_t trust_device_memory_held(void) { return (uint64_t)g_dev_held.load(); }
 
 extern "C" int trust_device_count(void)
 {

@@ -10,8 +10,12 @@
 // job's uint64 lands in its own control block, summed once at the end.
 #include <hip/hip_runtime.h>
 
+#include <sys/types.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -135,10 +139,116 @@ struct TileSlot {
 
 }  // namespace
 
+// ---- job journal (resume after an interrupted stream) -------------------
+// A text file: a header that names the partition, then one line per
+// finished job, appended and flushed as each job's count lands on the host:
+//   trust-tiles-journal 1 n=<n> tiles=<fingerprint>
+//   job <id> <i> <j> <k> <triangles> <probes>
+// Counts are additive, so a resumed stream counts only the jobs the file
+// lacks.  A last line without its newline (a write cut short) is ignored.
+struct Journal {
+    FILE *f = nullptr;
+    std::vector<char> have;                // job id -> already in the file
+    std::vector<unsigned long long> cnt, prb;
+    ~Journal()
+    {
+        if (f)
+            fclose(f);
+    }
+};
+
+static uint64_t tiles_fingerprint(const trust_csr *tiles, int n)
+{
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over sizes and edge entries
+    auto mix = [&h](uint64_t x) {
+        for (int b = 0; b < 8; b++) {
+            h ^= (x >> (8 * b)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    for (int t = 0; t < n * n; t++) {
+        const trust_csr &c = tiles[t];
+        mix((uint64_t)c.vertices);
+        mix((uint64_t)c.edges);
+        const int64_t step = std::max<int64_t>(1, c.edges / 64);
+        for (int64_t e = 0; e < c.edges; e += step)
+            mix((uint64_t)(uint32_t)c.adj[e]);
+    }
+    return h;
+}
+
+static int journal_open(Journal &jr, const char *path, int n, const trust_csr *tiles,
+                        long long njobs)
+{
+    char head[128];
+    snprintf(head, sizeof(head), "trust-tiles-journal 1 n=%d tiles=%016llx", n,
+             (unsigned long long)tiles_fingerprint(tiles, n));
+    jr.have.assign((size_t)njobs, 0);
+    jr.cnt.assign((size_t)njobs, 0);
+    jr.prb.assign((size_t)njobs, 0);
+    std::string text;
+    if (FILE *in = fopen(path, "rb")) {
+        char buf[1 << 16];
+        size_t got;
+        while ((got = fread(buf, 1, sizeof(buf), in)) > 0)
+            text.append(buf, got);
+        fclose(in);
+    }
+    bool fresh = text.empty();
+    if (!fresh) {
+        const size_t eol = text.find('\n');
+        if (eol == std::string::npos || text.compare(0, eol, head) != 0)
+            return fail(TRUST_ERR_FORMAT, "journal %s belongs to another partition (expected \"%s\")",
+                        path, head);
+        size_t at = eol + 1;
+        while (at < text.size()) {
+            const size_t end = text.find('\n', at);
+            if (end == std::string::npos)
+                break;  // a line cut short: that job is counted again
+            long long id;
+            int i, j, k;
+            unsigned long long c, w;
+            if (sscanf(text.c_str() + at, "job %lld %d %d %d %llu %llu", &id, &i, &j, &k, &c, &w) != 6 ||
+                id < 0 || id >= njobs)
+                return fail(TRUST_ERR_FORMAT, "journal %s: bad line at byte %zu", path, at);
+            jr.have[(size_t)id] = 1;
+            jr.cnt[(size_t)id] = c;
+            jr.prb[(size_t)id] = w;
+            at = end + 1;
+        }
+        if (at < text.size()) {  // drop the partial line before appending
+            if (truncate(path, (off_t)at) != 0)
+                return fail(TRUST_ERR_IO, "journal %s: cannot drop a partial line", path);
+        }
+    }
+    jr.f = fopen(path, "ab");
+    if (!jr.f)
+        return fail(TRUST_ERR_IO, "cannot open journal %s", path);
+    if (fresh && (fprintf(jr.f, "%s\n", head) < 0 || fflush(jr.f) != 0))
+        return fail(TRUST_ERR_IO, "cannot write journal %s", path);
+    return TRUST_OK;
+}
+
+static int journal_add(Journal &jr, long long id, int n, unsigned long long c, unsigned long long w)
+{
+    const int i = (int)(id / ((long long)n * n)), j = (int)(id / n % n), k = (int)(id % n);
+    if (fprintf(jr.f, "job %lld %d %d %d %llu %llu\n", id, i, j, k, c, w) < 0 || fflush(jr.f) != 0)
+        return fail(TRUST_ERR_IO, "cannot append to the journal");
+    jr.have[(size_t)id] = 1;
+    jr.cnt[(size_t)id] = c;
+    jr.prb[(size_t)id] = w;
+    return TRUST_OK;
+}
+
 // Streams all jobs of an n x n partition; tiles[i*n+j] is tile (i,j).
-// job_counts (nullable, n^3 entries) receives each job's count.
+// job_counts (nullable, n^3 entries) receives each job's count.  With a
+// journal path, jobs already in the journal are skipped and each new job is
+// appended as its count lands; max_new_jobs > 0 stops after that many new
+// jobs, and *remaining (nullable) receives the jobs still missing.
 static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_params &p,
-                        int cache_slots, uint64_t *job_counts, trust_result *r)
+                        int cache_slots, uint64_t *job_counts, trust_result *r,
+                        const char *journal = nullptr, long long max_new_jobs = 0,
+                        long long *remaining = nullptr)
 {
     const long long njobs = (long long)n * n * n;
     bool off64 = (p.flags & TRUST_FLAG_OFFSETS64) != 0;
@@ -238,8 +348,65 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
             return rc;
         HIPCHK(hipMemcpy(degs.p, d.data(), sizeof(int32_t) * d.size(), hipMemcpyHostToDevice));
     }
+    Journal jr;
+    if (journal && (rc = journal_open(jr, journal, n, tiles, njobs)))
+        return rc;
+    std::vector<long long> todo;  // the jobs this call counts, in stream order
+    for (long long job = 0; job < njobs; job++)
+        if (!journal || !jr.have[(size_t)job])
+            todo.push_back(job);
+    if (max_new_jobs > 0 && (long long)todo.size() > max_new_jobs)
+        todo.resize((size_t)max_new_jobs);
     DevBuf ctls;
     const long long nctl = short_walk ? 2 * njobs : njobs;
+    const int per_job = short_walk ? 2 : 1;  // control blocks per job
+    // journal: each job's control blocks come back to pinned host memory
+    // right after its launches, with an event to poll
+    TcCtl *hctl = nullptr;
+    std::vector<hipEvent_t> cev;
+    struct JournalGuard {
+        TcCtl *&h;
+        std::vector<hipEvent_t> &e;
+        hipStream_t s;
+        ~JournalGuard()
+        {
+            if (h || !e.empty())
+                (void)hipStreamSynchronize(s);
+            for (auto x : e)
+                if (x)
+                    (void)hipEventDestroy(x);
+            if (h)
+                (void)hipHostFree(h);
+        }
+    } journal_guard{hctl, cev, ctx->stream};
+    if (journal) {
+        HIPCHK(hipHostMalloc((void **)&hctl, sizeof(TcCtl) * (size_t)nctl, hipHostMallocDefault));
+        cev.assign((size_t)njobs, nullptr);
+        for (long long job : todo)
+            HIPCHK(hipEventCreateWithFlags(&cev[(size_t)job], hipEventDisableTiming));
+    }
+    size_t journaled = 0;  // todo[0, journaled) are in the file
+    auto journal_ready = [&](bool wait) -> int {
+        while (journaled < todo.size()) {
+            const long long job = todo[journaled];
+            if (!cev[(size_t)job])
+                break;
+            hipError_t e = wait ? hipEventSynchronize(cev[(size_t)job]) : hipEventQuery(cev[(size_t)job]);
+            if (e == hipErrorNotReady)
+                break;
+            if (e != hipSuccess)
+                return hip_fail(e, "journal event");
+            unsigned long long c = 0, w = 0;
+            for (int q = 0; q < per_job; q++) {
+                c += hctl[(size_t)(per_job * job + q)].count;
+                w += hctl[(size_t)(per_job * job + q)].probes;
+            }
+            if (int rc2 = journal_add(jr, job, n, c, w))
+                return rc2;
+            journaled++;
+        }
+        return TRUST_OK;
+    };
     rc = ctls.ensure(sizeof(TcCtl) * (size_t)nctl);
     if (rc)
         return rc;
@@ -317,12 +484,13 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                                               nullptr, false, ctx->stream, nullptr, nullptr);
     };
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-    for (long long job = 0; job < njobs; job++) {
+    for (size_t t = 0; t < todo.size(); t++) {
+        const long long job = todo[t];
         const int ji = (int)(job / ((long long)n * n)), jj = (int)(job / n % n), jk = (int)(job % n);
         int cur[3], nxt[3] = {-1, -1, -1};
         job_tiles(job, cur);
-        if (job + 1 < njobs)
-            job_tiles(job + 1, nxt);
+        if (t + 1 < todo.size())
+            job_tiles(todo[t + 1], nxt);
         if (short_walk && walk_row[(size_t)jk] != ji) {
             // the walk of tile (i,k), built once for the jobs (i, *, k)
             const int wt = ji * n + jk;
@@ -386,11 +554,19 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         }
         (void)jj;
         HIPCHK(hipEventRecord(jev[(size_t)(2 * job + 1)], ctx->stream));
+        if (journal) {
+            const size_t c0 = (size_t)(per_job * job);
+            HIPCHK(hipMemcpyAsync(hctl + c0, (TcCtl *)ctls.p + c0, sizeof(TcCtl) * per_job,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipEventRecord(cev[(size_t)job], ctx->stream));
+            if ((rc = journal_ready(false)))
+                return rc;
+        }
         for (int q = 0; q < 3; q++)
             if (sl[q] >= 0)
                 HIPCHK(hipEventRecord(slots[sl[q]].released, ctx->stream));
         // Prefetch the next job's tiles while this one counts.
-        if (job + 1 < njobs) {
+        if (t + 1 < todo.size()) {
             const int keep2[6] = {cur[0], cur[1], cur[2], nxt[0], nxt[1], nxt[2]};
             for (int q = 0; q < 3; q++) {
                 int dummy;
@@ -399,7 +575,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
                     break;  // too few slots to prefetch without evicting the running job
                 if (nxt[q] < 0)
                     continue;
-                rc = acquire(nxt[q], job + 1, keep2, dummy);
+                rc = acquire(nxt[q], todo[t + 1], keep2, dummy);
                 if (rc)
                     return rc;
             }
@@ -410,7 +586,7 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
     HIPCHK(hipMemcpyAsync(h.data(), ctls.p, sizeof(TcCtl) * (size_t)nctl, hipMemcpyDeviceToHost,
                           ctx->stream));
     if (short_walk)  // fold each job's in-walk count into its out-walk count
-        for (long long j = 0; j < njobs; j++) {
+        for (long long j : todo) {
             TcCtl &a = h[(size_t)(2 * j)];
             const TcCtl &b = h[(size_t)(2 * j + 1)];
             a.count += b.count;
@@ -418,13 +594,17 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
             a.rebuilds += b.rebuilds;
         }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (journal && (rc = journal_ready(true)))
+        return rc;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     double count_s = 0;
-    for (long long j = 0; j < njobs; j++) {
+    std::vector<char> counted((size_t)njobs, 0);
+    for (long long j : todo) {
         float jm = 0;
         HIPCHK(hipEventElapsedTime(&jm, jev[(size_t)(2 * j)], jev[(size_t)(2 * j + 1)]));
         count_s += jm * 1e-3;
+        counted[(size_t)j] = 1;
     }
     double build_s = 0;
     for (int b = 0; b < nbuilds; b++) {
@@ -433,14 +613,26 @@ static int stream_tiles(Ctx *ctx, const trust_csr *tiles, int n, const trust_par
         build_s += bm * 1e-3;
     }
     uint64_t total = 0, probes = 0, rebuilds = 0;
+    long long missing = 0;
     for (long long j = 0; j < njobs; j++) {
-        const TcCtl &x = h[(size_t)(short_walk ? 2 * j : j)];
-        total += x.count;
-        probes += x.probes;
-        rebuilds += x.rebuilds;
+        uint64_t c = 0;
+        if (counted[(size_t)j]) {  // this call
+            const TcCtl &x = h[(size_t)(short_walk ? 2 * j : j)];
+            c = x.count;
+            probes += x.probes;
+            rebuilds += x.rebuilds;
+        } else if (journal && jr.have[(size_t)j]) {  // an earlier call
+            c = jr.cnt[(size_t)j];
+            probes += jr.prb[(size_t)j];
+        } else {
+            missing++;
+        }
+        total += c;
         if (job_counts)
-            job_counts[j] = x.count;
+            job_counts[j] = c;
     }
+    if (remaining)
+        *remaining = missing;
     if (r) {
         memset(r, 0, sizeof(*r));
         r->vertices = Vsum;
@@ -482,6 +674,28 @@ extern "C" int trust_count_tiles(const trust_csr *tiles, int n, const trust_para
     // reserved[0]: device tile-cache slots (0 = 2n+2, enough to prefetch)
     int slots = p->reserved[0] > 0 ? p->reserved[0] : 2 * n + 2;
     return stream_tiles(&ctx, tiles, n, *p, slots, nullptr, r);
+}
+
+extern "C" int trust_count_tiles_journal(const trust_csr *tiles, int n, const trust_params *p,
+                                         const char *journal, int64_t max_new_jobs,
+                                         uint64_t *job_counts, trust_result *r)
+{
+    if (!tiles || !p || !journal || n < 1)
+        return fail(TRUST_ERR_ARG, "bad arguments");
+    int rc = check_params(*p);
+    if (rc)
+        return rc;
+    Ctx ctx;
+    rc = ctx_init(&ctx, p->device);
+    if (rc)
+        return rc;
+    int slots = p->reserved[0] > 0 ? p->reserved[0] : 2 * n + 2;
+    long long remaining = 0;
+    rc = stream_tiles(&ctx, tiles, n, *p, slots, job_counts, r, journal, (long long)max_new_jobs,
+                      &remaining);
+    if (rc)
+        return rc;
+    return (int)std::min<long long>(remaining, INT32_MAX);
 }
 
 extern "C" int trust_triangle_count_partition(int rank, const char *folder, int total_process,
