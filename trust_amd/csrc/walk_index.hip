@@ -47,9 +47,12 @@
 
 namespace trust {
 
-constexpr int kBinLg = 14;
-constexpr int kBinSize = 1 << kBinLg;  // targets per bin (C1's LDS counters: 64 KB)
-constexpr int kMaxBins = 8192;         // V <= 2^27 (pass B's LDS cursors: <= 64 KB)
+// 2^13 targets per bin: a bin's rows (about 1.6 MB of entries at R-MAT
+// scale 26) stay in one XCD's 4 MB L2 while C2 fills them, with room for
+// the next bin (2^14: 44.0 ms for the whole build at scale 26, 2^13: 40.6)
+constexpr int kBinLg = 13;
+constexpr int kBinSize = 1 << kBinLg;  // targets per bin (C1's LDS counters: 32 KB)
+constexpr int kMaxBins = 16384;        // V <= 2^27 (pass B's LDS cursors: <= 64 KB, 128 with int64)
 constexpr int kWin = 4;                // 64-wide windows per step of passes A and B
 constexpr int kPassThreads = 1024;
 constexpr int kPassWaves = kPassThreads / kWave;
@@ -291,7 +294,7 @@ wi_rank_kernel(const WiSlice *__restrict__ sl, const Rec *__restrict__ recs,
 // 8-byte entry at a time: random stores at about 20 G/s (39 ms at R-MAT
 // scale 26).  So a workgroup takes a chunk of whole C1 rounds, whose records
 // of any one target hold consecutive slots, sorts them by target in LDS (a
-// counting sort over the bin's 16k targets) and stores them in that order:
+// counting sort over the bin's 8k targets) and stores them in that order:
 // the records of a target go out together, to consecutive addresses.  The
 // chunks of a bin are queued on one XCD (bins dealt largest first to the
 // least loaded XCD); a workgroup reads the hardware XCC_ID and drains its
@@ -307,7 +310,7 @@ struct PlaceStage {
 template <typename OffT, bool kTile>
 constexpr int place_chunk()
 {
-    // LDS: 16k counters (64 KB) + one chunk of staged records (<= 64 KB)
+    // LDS: 8k counters (32 KB) + one chunk of staged records (<= 64 KB)
     return sizeof(PlaceStage<OffT, kTile>) <= 8 ? 8192
            : sizeof(PlaceStage<OffT, kTile>) <= 16 ? 4096 : 2048;
 }
@@ -375,7 +378,7 @@ wi_place_kernel(const WiSlice *__restrict__ sl, const WiChunk *__restrict__ chun
             }
         }
         __syncthreads();
-        // exclusive scan of the 16k counts: each thread its kCntPer, then the
+        // exclusive scan of the 8k counts: each thread its kCntPer, then the
         // per-thread totals across the workgroup
         int run = 0, mine[kCntPer];
 #pragma unroll
