@@ -720,13 +720,15 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
             return rc;
     }
     hipEvent_t e1 = timed ? ctx->ev[1] : nullptr, e2 = timed ? ctx->ev[2] : nullptr;
+    // the short-side walk probes long lists: six windows in flight per wave
+    const int m = R.side && (mode == kModeCount || mode == kModeGather) ? mode | kProbeDeep : mode;
     if (g.off64)
-        rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, mode,
+        rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, m,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, R.learned,
                                     ctx->stream, e1, e2);
     else
-        rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, mode,
+        rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, m,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, R.learned,
                                     ctx->stream, e1, e2);

@@ -18,9 +18,9 @@
 //  * The probe walk takes 64 neighbours v of u at a time: lanes load
 //    (beg[v], deg v), an inclusive wave scan (DPP row_shr / row_bcast) lays
 //    the 64 lists end to end, and each 64-wide window of that concatenation
-//    is one coalesced load per lane.  kUnroll windows are in flight per
-//    wave before any hash lookup, so each lane has kUnroll independent
-//    loads outstanding.  The segment owning each lane is found with a
+//    is one coalesced load per lane.  A batch of kU windows (four, or six
+//    for the short-side walk's long lists) is in flight per wave before
+//    any hash lookup, so each lane has kU independent loads outstanding.  The segment owning each lane is found with a
 //    64-entry LDS mark array and a wave max-scan; a window inside a single
 //    list skips that and reads its base with one readlane.  This replaces
 //    the reference's per-lane skip loop with its 32-lane __shfl_sync
@@ -180,7 +180,7 @@ __device__ __forceinline__ bool cuckoo_insert(int *tab, int lgb, int key)
     return true;
 }
 
-// Membership count of kUnroll keys (kNoProbe matches no slot), summed over
+// Membership count of N keys (kNoProbe matches no slot), summed over
 // the wave: the per-key hit masks are counted with scalar popcounts, which
 // keeps vector instructions (the walk's bound) for the lookup itself.  All
 // 64 lanes must call it.  Both buckets of every key are read at once:
@@ -247,7 +247,7 @@ __device__ __forceinline__ void table_insert(int *tab, int lg, int key)
     }
 }
 
-// Membership of up to kUnroll keys (key < 0: skip), summed over the wave.
+// Membership of up to N keys (key < 0: skip), summed over the wave.
 // First slots of all keys are read together, then each key finishes its
 // probe sequence.  All 64 lanes must call it.
 template <bool kGlobal, int N>
@@ -339,7 +339,7 @@ __device__ __forceinline__ Group<OffT> walk_group(const TcArgs<OffT> &a, OffT at
 // concatenation against the table and returns the hits of the whole wave
 // (wave-uniform; gather mode: the wave's checksum).  All 64 lanes must call
 // it.  `mark` is this wave's 64-int LDS scratch: all-zero on entry and exit.
-template <typename OffT, bool kGlobal, int kMode = kModeCount>
+template <typename OffT, bool kGlobal, int kMode, int kU>
 __device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g, int wbeg, int wend,
                                                       const int32_t *__restrict__ adjP,
                                                       const int *tab, int lg, bool linear,
@@ -382,6 +382,19 @@ __device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g
 #pragma unroll
             for (int k = 0; k < N; k++)
                 checksum += w[k] >= 0 ? (unsigned int)w[k] : 0u;
+        } else if constexpr (N > 4 && N % 2 == 0) {
+            // two halves, so that the table reads of one half and their
+            // compares need only half the registers
+            int lo[N / 2], hi[N / 2];
+#pragma unroll
+            for (int k = 0; k < N / 2; k++) {
+                lo[k] = w[k];
+                hi[k] = w[N / 2 + k];
+            }
+            total += linear ? linear_count<kGlobal, N / 2>(tab, lg, lo)
+                            : cuckoo_count<kGlobal, N / 2>(tab, lg, lo);
+            total += linear ? linear_count<kGlobal, N / 2>(tab, lg, hi)
+                            : cuckoo_count<kGlobal, N / 2>(tab, lg, hi);
         } else {
             total += linear ? linear_count<kGlobal, N>(tab, lg, w)
                             : cuckoo_count<kGlobal, N>(tab, lg, w);
@@ -404,59 +417,59 @@ __device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g
             else
                 return adjP[idx];
         };
-        auto bases = [&](int at, OffT (&bj)[kUnroll]) {
+        auto bases = [&](int at, OffT (&bj)[kU]) {
 #pragma unroll
-            for (int k = 0; k < kUnroll; k++)
+            for (int k = 0; k < kU; k++)
                 bj[k] = at + k * kWave < bend ? window_base(at + k * kWave) : (OffT)0;
         };
         // The loads of a batch lying entirely before bend.  Every lane loads
         // and nothing reads the results here: with no branch around the
-        // loads the compiler knows exactly kUnroll loads are younger than the
+        // loads the compiler knows exactly kU loads are younger than the
         // previous batch and waits for that batch alone.
-        auto fetch_full = [&](int at, int (&w)[kUnroll]) {
-            OffT bj[kUnroll];
+        auto fetch_full = [&](int at, int (&w)[kU]) {
+            OffT bj[kU];
             bases(at, bj);
 #pragma unroll
-            for (int k = 0; k < kUnroll; k++)
+            for (int k = 0; k < kU; k++)
                 w[k] = load(bj[k] + (OffT)(at + k * kWave + lane));
         };
         // The last batch: positions past bend read adjP[0] (which exists,
         // since bend > 0) and are masked before the lookup.
-        auto fetch_tail = [&](int at, int (&w)[kUnroll]) {
-            OffT bj[kUnroll];
+        auto fetch_tail = [&](int at, int (&w)[kU]) {
+            OffT bj[kU];
             bases(at, bj);
 #pragma unroll
-            for (int k = 0; k < kUnroll; k++) {
+            for (int k = 0; k < kU; k++) {
                 const int t = at + k * kWave + lane;
                 w[k] = load(t < bend ? bj[k] + (OffT)t : (OffT)0);
             }
         };
-        // Software pipeline over batches of kUnroll windows: the next
+        // Software pipeline over batches of kU windows: the next
         // batch's loads are in flight while the current one is looked up.
-        constexpr int kBatch = kWave * kUnroll;
+        constexpr int kBatch = kWave * kU;
         const int last = b + (bend - b - 1) / kBatch * kBatch;  // start of the last batch
-        int wcur[kUnroll];
+        int wcur[kU];
         if (last == b) {
             fetch_tail(b, wcur);
         } else {
             fetch_full(b, wcur);
             for (int bn = b + kBatch; bn < last; bn += kBatch) {
-                int wnext[kUnroll];
+                int wnext[kU];
                 fetch_full(bn, wnext);
                 lookup(wcur);
 #pragma unroll
-                for (int k = 0; k < kUnroll; k++)
+                for (int k = 0; k < kU; k++)
                     wcur[k] = wnext[k];
             }
-            int wtail[kUnroll];
+            int wtail[kU];
             fetch_tail(last, wtail);
             lookup(wcur);
 #pragma unroll
-            for (int k = 0; k < kUnroll; k++)
+            for (int k = 0; k < kU; k++)
                 wcur[k] = wtail[k];
         }
 #pragma unroll
-        for (int k = 0; k < kUnroll; k++)
+        for (int k = 0; k < kU; k++)
             wcur[k] = last + k * kWave + lane < bend ? wcur[k] : kNoProbe;
         lookup(wcur);
     }
@@ -469,7 +482,7 @@ __device__ __forceinline__ unsigned long long probe_windows(const Group<OffT> &g
 // Whole workgroup counts vertex u with a table at `tab` (LDS or HBM) of
 // capacity 2^cap_lg.  misc[0]: unit cursor, misc[2]: window total;
 // ctot: kMaxChunks+1 ints of LDS.
-template <typename OffT, bool kGlobal, int kMode>
+template <typename OffT, bool kGlobal, int kMode, int kU>
 __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *tab, int cap_lg,
                                              int *mark, int *misc, int *ctot,
                                              unsigned long long &acc, unsigned long long &probes)
@@ -557,10 +570,10 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
         }
         __syncthreads();
         const int TW = misc[2];
-        // units of K windows, K a multiple of kUnroll
+        // units of K windows, K a multiple of kU
         int K = TW / (nwaves * 4);
-        K = K < kUnroll ? kUnroll : (K > 64 ? 64 : K);
-        K = (K + kUnroll - 1) / kUnroll * kUnroll;
+        K = K < kU ? kU : (K > 64 ? 64 : K);
+        K = (K + kU - 1) / kU * kU;
         int cur = -1;
         Group<OffT> g = {0, 0, 0, 0, 0};
         for (;;) {
@@ -587,7 +600,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
                     cur = lo;
                 }
                 const int cend = min(w1, ctot[lo + 1]);
-                hits += probe_windows<OffT, kGlobal, kMode>(g, w0 - ctot[lo], cend - ctot[lo], a.adjP,
+                hits += probe_windows<OffT, kGlobal, kMode, kU>(g, w0 - ctot[lo], cend - ctot[lo], a.adjP,
                                                      tab, lg, linear, mark);
                 w0 = cend;
             }
@@ -604,7 +617,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
             const int idx = c * kWave + lane;
             const Group<OffT> g = walk_group(a, sW + (OffT)idx, idx < dW);
             wave_probes += (unsigned long long)g.total;
-            hits += probe_windows<OffT, kGlobal, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab,
+            hits += probe_windows<OffT, kGlobal, kMode, kU>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab,
                                                  lg, linear, mark);
         }
     }
@@ -620,7 +633,7 @@ __device__ __forceinline__ void group_vertex(const TcArgs<OffT> &a, int u, int *
 }
 
 // One wavefront counts vertex u with its own LDS table slice.
-template <typename OffT, int kMode>
+template <typename OffT, int kMode, int kU>
 __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *tab, int *mark,
                                             unsigned long long &acc, unsigned long long &probes)
 {
@@ -683,7 +696,7 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
             g = load_group(v, valid, a.begP);
         }
         wave_probes += (unsigned long long)g.total;
-        hits += probe_windows<OffT, false, kMode>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab, lg,
+        hits += probe_windows<OffT, false, kMode, kU>(g, 0, (g.total + kWave - 1) / kWave, a.adjP, tab, lg,
                                            linear, mark);
     }
     wave_sync();
@@ -698,7 +711,7 @@ __device__ __forceinline__ void wave_vertex(const TcArgs<OffT> &a, int u, int *t
 }
 
 // ------------------------------------------------------------ the kernel
-template <typename OffT, int kMode, bool kGlobal>
+template <typename OffT, int kMode, bool kGlobal, int kU>
 __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int *tab, int cap_lg,
                                             int *mark, int *misc, int *ctot,
                                             unsigned long long &acc, unsigned long long &probes)
@@ -717,7 +730,7 @@ __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int
             break;
         const long long last = min(first + (long long)a.chunk_group, le);
         for (long long i = first; i < last; i++)
-            group_vertex<OffT, kGlobal, kMode>(a, a.list[i], tab, cap_lg, mark, misc, ctot, acc,
+            group_vertex<OffT, kGlobal, kMode, kU>(a, a.list[i], tab, cap_lg, mark, misc, ctot, acc,
                                                 probes);
     }
 }
@@ -735,7 +748,7 @@ __device__ __forceinline__ void group_phase(const TcArgs<OffT> &a, int tier, int
 template <typename OffT>
 constexpr int kMinWavesPerEU = sizeof(OffT) == 4 ? 8 : 7;
 
-template <typename OffT, int kMode, bool kBig>
+template <typename OffT, int kMode, bool kBig, int kU = kUnroll>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEU<OffT>, 8)))
 tc_count_kernel(TcArgs<OffT> a)
 {
@@ -759,10 +772,10 @@ tc_count_kernel(TcArgs<OffT> a)
         int cap_lg = 0;
         while ((2ll << cap_lg) <= a.spill_slots)
             cap_lg++;
-        group_phase<OffT, kMode, true>(a, a.spill_tier, tab, cap_lg, mark, misc, ctot, acc, probes);
+        group_phase<OffT, kMode, true, kU>(a, a.spill_tier, tab, cap_lg, mark, misc, ctot, acc, probes);
     }
     if (a.group_tier >= 0)
-        group_phase<OffT, kMode, false>(a, a.group_tier, lds, a.group_lg, mark, misc, ctot, acc,
+        group_phase<OffT, kMode, false, kU>(a, a.group_tier, lds, a.group_lg, mark, misc, ctot, acc,
                                         probes);
     __syncthreads();  // the wave tables reuse the group table's LDS
     if (!kBig && a.wave_tier >= 0) {
@@ -806,7 +819,7 @@ tc_count_kernel(TcArgs<OffT> a)
             }
             const long long last = min(first + (long long)a.chunk_wave, le);
             for (long long i = first; i < last; i++)
-                wave_vertex<OffT, kMode>(a, a.list[i], wtab, mark, acc, probes);
+                wave_vertex<OffT, kMode, kU>(a, a.list[i], wtab, mark, acc, probes);
         }
     }
     if constexpr (kMode == kModeBatch)
@@ -1245,15 +1258,23 @@ template <typename OffT, bool kBig>
 static void launch_count_t(const TcArgs<OffT> &a, int blocks, int threads, size_t lds_bytes,
                            int mode, hipStream_t stream)
 {
+    const bool deep = (mode & kProbeDeep) != 0;
+    mode &= ~kProbeDeep;
     if (mode == kModeInstr)
         hipLaunchKernelGGL((tc_count_kernel<OffT, kModeInstr, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
+    else if (mode == kModeGather && deep)
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeGather, kBig, kUnrollDeep>), dim3(blocks),
+                           dim3(threads), lds_bytes, stream, a);
     else if (mode == kModeGather)
         hipLaunchKernelGGL((tc_count_kernel<OffT, kModeGather, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
     else if (mode == kModeBatch)
         hipLaunchKernelGGL((tc_count_kernel<OffT, kModeBatch, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
+    else if (deep)
+        hipLaunchKernelGGL((tc_count_kernel<OffT, kModeCount, kBig, kUnrollDeep>), dim3(blocks),
+                           dim3(threads), lds_bytes, stream, a);
     else
         hipLaunchKernelGGL((tc_count_kernel<OffT, kModeCount, kBig>), dim3(blocks), dim3(threads),
                            lds_bytes, stream, a);
@@ -1273,17 +1294,26 @@ hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t l
 template <typename OffT, bool kBig>
 static hipError_t count_occupancy_t(int threads, size_t lds_bytes, int mode, int *blocks_per_cu)
 {
+    const bool deep = (mode & kProbeDeep) != 0;
+    mode &= ~kProbeDeep;
     if (mode == kModeInstr)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             blocks_per_cu, tc_count_kernel<OffT, kModeInstr, kBig>, threads, lds_bytes);
     if (mode == kModeGather)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, tc_count_kernel<OffT, kModeGather, kBig>, threads, lds_bytes);
+        return deep ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          blocks_per_cu, tc_count_kernel<OffT, kModeGather, kBig, kUnrollDeep>,
+                          threads, lds_bytes)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          blocks_per_cu, tc_count_kernel<OffT, kModeGather, kBig>, threads,
+                          lds_bytes);
     if (mode == kModeBatch)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             blocks_per_cu, tc_count_kernel<OffT, kModeBatch, kBig>, threads, lds_bytes);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, tc_count_kernel<OffT, kModeCount, kBig>, threads, lds_bytes);
+    return deep ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      blocks_per_cu, tc_count_kernel<OffT, kModeCount, kBig, kUnrollDeep>, threads,
+                      lds_bytes)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      blocks_per_cu, tc_count_kernel<OffT, kModeCount, kBig>, threads, lds_bytes);
 }
 
 template <typename OffT>
