@@ -254,15 +254,25 @@ def bench_rmat(args, T):
     index_s, walk = first.index_seconds, first.walk_used
     log(f"walk: {'short-side' if walk else 'reference'}; per-graph index {index_s * 1e3:.1f} ms")
     if args.only_head:
+        # the headline's own steps: the same training and judge, then the
+        # AUTO schedule on fresh graphs (a kernel trace of this run sees the
+        # judge's four counts first, then warmup + steps headline counts)
         hkw = dict(base_kw, schedule=T.SCHED_DEGREE, flags=fresh)
+        if args.schedule == "learned":
+            ctx.train_cost_model(epochs=15, max_samples=200000, seed=1, **base_kw)
+            hkw["schedule"] = T.SCHED_AUTO
         head = run_steps(lambda: ctx.count(**hkw), args.warmup, args.steps,
-                         "degree (head only), fresh graph", E)
+                         "headline only, fresh graph", E)
         r0 = head["r"]
         line = line_head(args, "rmat", E / head["dev_s"] / 1e9, "GTEPS", head["dev_s"] * 1e3)
         line["config"] = {"workload": f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}",
                           "V": V, "E": E, "triangles": r0.triangles, "W_probes": r0.probes,
                           "only_head": True, "index_ms": head["idx_s"] * 1e3,
+                          "plan_ms": r0.plan_seconds * 1e3,
                           "count_kernel_ms": head["cnt_s"] * 1e3,
+                          "schedule_used": "learned" if r0.schedule_used == T.SCHED_LEARNED
+                          else "degree",
+                          "head_counts": args.warmup + args.steps,
                           "alg_bytes_per_count": r0.alg_bytes}
         ctx.close()
         # one 2 GiB copy: the known-traffic kernel that calibrates byte counters

@@ -107,6 +107,38 @@ def test_walk_features_match_oracle(gpu, scale, ef, seed):
     assert np.allclose(f, oracle.mlp_scaled_features_short(beg, adj), rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_cost_predictions_match_oracle(gpu, rmat16, seed):
+    # the plan's inference (hidden layer on the matrix cores in f16 with f32
+    # accumulation) against the float64 oracle forward on the same features;
+    # vertices with d+ < 2 are not counted and cost 0
+    beg, adj, _ = rmat16
+    rng = np.random.default_rng(seed)
+    H = 32
+    p = rng.normal(0, 0.3, T.mlp_param_count()).astype(np.float32)
+    # last layer small around 0.5, so that outputs stay inside the [0, 1] clamp
+    p[4 * H + H * H + H: 4 * H + H * H + 2 * H] = rng.normal(0, 0.02, H)
+    p[-1] = 0.5
+    with T.Context(gpu) as c:
+        c.load(beg, adj)
+        c.set_cost_model(p)
+        feat = c.features()
+        cost = c.predict_costs().astype(np.float64)
+    d = np.diff(beg)
+    y = np.clip(oracle.mlp_forward(p.astype(np.float64), feat.astype(np.float64)), 0.0, 1.0)
+    counted = d >= 2
+    assert np.all(cost[~counted] == 0)
+    assert np.mean((y[counted] > 0) & (y[counted] < 1)) > 0.9  # the outputs vary
+    got = np.log2(1.0 + cost[counted]) / 32.0
+    # f16 operands (relative rounding 2^-11) on the 32-term hidden layer:
+    # about 1e-3 on y; a wrong operand layout would be off by O(1)
+    err = np.abs(got - y[counted])
+    print(f"[gpu] cost model inference vs oracle: max {err.max():.2e}, mean {err.mean():.2e}",
+          flush=True)
+    assert err.max() < 1.5e-2 and err.mean() < 2e-3
+    assert np.corrcoef(got, y[counted])[0, 1] > 0.999
+
+
 def test_learned_without_model_falls_back(gpu, rmat16):
     beg, adj, want = rmat16
     with T.Context(gpu) as c:

@@ -186,6 +186,20 @@ __global__ void __launch_bounds__(256) walk_feat_final_kernel(const OffT *__rest
 
 // Predicted single-wave cycles per vertex (0 for vertices that are not
 // counted); per-block partial sums of the predictions.
+//
+// The 32 x 32 hidden layer runs on the matrix cores.  A wave takes 32
+// vertices at a time and computes H2^T = W2 x H1^T (outputs x vertices) with
+// four v_mfma_f32_32x32x8_f16, f16 operands and f32 accumulation; the first
+// layer (3 inputs) and the last (one output) are vector code.  Operand
+// layout of that instruction, lane L, q = 0..3, r = 0..15:
+//   A[row L%32][k 4(L/32)+q],  B[k 4(L/32)+q][col L%32],
+//   D[row 8(r/4)+4(L/32)+r%4][col L%32]
+// so a column is a vertex: lanes L and L+32 share vertex L%32, each builds
+// half of its hidden units as its B operand and sums half of the output
+// rows, and one exchange adds the halves.
+typedef _Float16 mlp_half4 __attribute__((ext_vector_type(4)));
+typedef float mlp_float16 __attribute__((ext_vector_type(16)));
+
 template <typename OffT>
 __global__ void __launch_bounds__(256) cost_kernel(const OffT *__restrict__ beg, long long V,
                                                    const float *__restrict__ feat,
@@ -194,6 +208,7 @@ __global__ void __launch_bounds__(256) cost_kernel(const OffT *__restrict__ beg,
                                                    double *__restrict__ block_sum)
 {
     constexpr int H = kMlpHidden;
+    static_assert(H == 32, "one 32 x 32 MFMA tile per hidden layer");
     __shared__ float p[kMlpParams<H>];
     __shared__ double part[4];
     for (int i = threadIdx.x; i < kMlpParams<H>; i += blockDim.x)
@@ -201,37 +216,70 @@ __global__ void __launch_bounds__(256) cost_kernel(const OffT *__restrict__ beg,
     __syncthreads();
     const float *W1 = p, *b1 = p + 3 * H, *W2 = p + 4 * H, *b2 = W2 + H * H, *W3 = b2 + H,
                 *b3 = W3 + H;
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const int col = lane & 31, half = lane >> 5;
+    // A: W2 (output unit x input unit), the same for every tile
+    mlp_half4 a[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            a[c][q] = (_Float16)W2[col * H + 8 * c + 4 * half + q];
+    // this lane's output units: their bias and last-layer weight
+    float b2r[16], w3r[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int n = 8 * (r / 4) + 4 * half + (r % 4);
+        b2r[r] = b2[n];
+        w3r[r] = W3[n];
+    }
     double local = 0;
-    for (long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x; u < V;
-         u += (long long)gridDim.x * blockDim.x) {
-        const int d = (int)(beg[u + 1] - beg[u]);
-        float c = 0.f;
-        if (d >= 2) {
-            const float x0 = feat[3 * u], x1 = feat[3 * u + 1], x2 = feat[3 * u + 2];
-            float h1[H];
-#pragma unroll
-            for (int i = 0; i < H; i++) {
-                float a = b1[i] + W1[3 * i] * x0 + W1[3 * i + 1] * x1 + W1[3 * i + 2] * x2;
-                h1[i] = a > 0.f ? a : 0.f;
-            }
-            float y = b3[0];
-            for (int i = 0; i < H; i++) {
-                float a = b2[i];
-#pragma unroll
-                for (int j = 0; j < H; j++)
-                    a += W2[i * H + j] * h1[j];
-                y += W3[i] * (a > 0.f ? a : 0.f);
-            }
-            y = fminf(fmaxf(y, 0.f), 1.f);  // log2(1+cycles)/32 in [0, 1]
-            c = exp2f(y * 32.f) - 1.f;
+    const long long tiles = (V + 31) / 32;
+    for (long long t = (long long)blockIdx.x * (blockDim.x / kWave) + wave; t < tiles;
+         t += (long long)gridDim.x * (blockDim.x / kWave)) {
+        const long long u = t * 32 + col;
+        const bool in = u < V;
+        int d = 0;
+        float x0 = 0.f, x1 = 0.f, x2 = 0.f;
+        if (in) {
+            d = (int)(beg[u + 1] - beg[u]);
+            x0 = feat[3 * u];
+            x1 = feat[3 * u + 1];
+            x2 = feat[3 * u + 2];
         }
-        cost[u] = c;
-        local += (double)c;
+        // B: this lane's half of vertex u's first-layer units
+        mlp_half4 b[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k = 8 * c + 4 * half + q;
+                const float h = b1[k] + W1[3 * k] * x0 + W1[3 * k + 1] * x1 + W1[3 * k + 2] * x2;
+                b[c][q] = (_Float16)(h > 0.f ? h : 0.f);
+            }
+        mlp_float16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            acc[r] = b2r[r];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a[c], b[c], acc, 0, 0, 0);
+        float y = 0.f;
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            y += w3r[r] * (acc[r] > 0.f ? acc[r] : 0.f);
+        y += __shfl_xor(y, 32);
+        y = fminf(fmaxf(y + b3[0], 0.f), 1.f);  // log2(1+cycles)/32 in [0, 1]
+        const float cst = (in && d >= 2) ? exp2f(y * 32.f) - 1.f : 0.f;
+        if (in && half == 0) {
+            cost[u] = cst;
+            local += (double)cst;
+        }
     }
     for (int off = 32; off > 0; off >>= 1)
         local += __shfl_xor(local, off);
-    if (lane_id() == 0)
-        part[threadIdx.x / 64] = local;
+    if (lane == 0)
+        part[wave] = local;
     __syncthreads();
     if (threadIdx.x == 0)
         block_sum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
