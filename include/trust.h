@@ -440,6 +440,18 @@ int trust_partition_write(const char *folder, const trust_csr *csr, int n);
  * the reference (partition.cpp:72-121) holds the whole graph and every tile
  * in memory and stops at n <= 10 (:74-77); this takes n <= 1024. */
 int trust_partition_folder(const char *folder, int n);
+/* The R-MAT stream of trust_gen_rmat, preprocessed out of core straight from
+ * the generator into out_folder/begin.bin and adjacent.bin: byte-identical to
+ * trust_preprocess_edges on trust_gen_rmat's pairs, with the pairs, the
+ * oriented rows and the final adjacency passing through bucket files of at
+ * most bucket_pairs 8-byte keys under work_dir (removed at the end).  Host
+ * memory: arrays over the vertices (about 60 bytes each) plus two buckets.
+ * Followed by trust_partition_folder, it writes the 2D tiles of a graph that
+ * never sits in host memory whole; the reference holds the whole edge list
+ * and CSR (Preprocess/fromDirectToUndirect.cpp, preprocess_outdegree.cpp,
+ * partition.cpp:72-121). */
+int trust_rmat_csr_ooc(int scale, int64_t m, double a, double b, double c, uint64_t seed,
+                       const char *out_folder, const char *work_dir, int64_t bucket_pairs);
 
 /* ---- seeded synthetic generators (edge lists, not yet preprocessed) --- */
 /* No counterpart in the reference, which downloads its datasets

@@ -157,6 +157,8 @@ SIGNATURES = {
     "trust_partition_csr": (C.c_int, [C.POINTER(Csr), C.c_int, C.POINTER(Csr)]),
     "trust_partition_write": (C.c_int, [C.c_char_p, C.POINTER(Csr), C.c_int]),
     "trust_partition_folder": (C.c_int, [C.c_char_p, C.c_int]),
+    "trust_rmat_csr_ooc": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                     C.c_uint64, C.c_char_p, C.c_char_p, C.c_int64]),
     "trust_gen_rmat": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
                                  C.c_uint64, _P32, _P32]),
     "trust_gen_chung_lu": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_uint64,
@@ -314,6 +316,15 @@ def partition_write(folder: str, beg, adj, n: int):
 
 # ---------------------------------------------------------------- generators
 RMAT_ABC = (0.57, 0.19, 0.19)
+
+
+def rmat_csr_ooc(scale: int, edge_factor: int, seed: int, out_folder: str, work_dir: str,
+                 bucket_pairs: int = 1 << 27, abc=RMAT_ABC) -> None:
+    """gen_rmat + preprocess_edges out of core: writes out_folder/begin.bin and
+    adjacent.bin through bucket files of at most bucket_pairs keys in work_dir."""
+    m = edge_factor << scale
+    _check(lib().trust_rmat_csr_ooc(scale, m, abc[0], abc[1], abc[2], seed,
+                                    os.fsencode(out_folder), os.fsencode(work_dir), bucket_pairs))
 
 
 def gen_rmat(scale: int, edge_factor: int = 16, seed: int = 1, abc=RMAT_ABC):

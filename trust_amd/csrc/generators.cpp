@@ -34,6 +34,31 @@ struct Stream {
 
 using namespace trust;
 
+namespace trust {
+
+// Pairs [e0, e0 + count) of the R-MAT stream (pair e depends only on seed
+// and e, so any range can be drawn alone).
+void gen_rmat_range(int scale, double a, double b, double c, uint64_t seed, int64_t e0,
+                    int64_t count, int32_t *src, int32_t *dst)
+{
+    const double ab = a + b, abc = a + b + c;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < count; i++) {
+        Stream r(seed, (uint64_t)(e0 + i));
+        uint32_t u = 0, v = 0;
+        for (int l = 0; l < scale; l++) {
+            double x = r.uniform();
+            uint32_t bu = x >= ab, bv = (x >= a && x < ab) || x >= abc;
+            u = (u << 1) | bu;
+            v = (v << 1) | bv;
+        }
+        src[i] = (int32_t)u;
+        dst[i] = (int32_t)v;
+    }
+}
+
+}  // namespace trust
+
 extern "C" int trust_gen_rmat(int scale, int64_t m, double a, double b, double c, uint64_t seed,
                               int32_t *src, int32_t *dst)
 {
@@ -42,20 +67,7 @@ extern "C" int trust_gen_rmat(int scale, int64_t m, double a, double b, double c
                     (long long)m);
     if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
         return fail(TRUST_ERR_ARG, "trust_gen_rmat: bad quadrant probabilities");
-    const double ab = a + b, abc = a + b + c;
-#pragma omp parallel for schedule(static)
-    for (int64_t e = 0; e < m; e++) {
-        Stream r(seed, (uint64_t)e);
-        uint32_t u = 0, v = 0;
-        for (int l = 0; l < scale; l++) {
-            double x = r.uniform();
-            uint32_t bu = x >= ab, bv = (x >= a && x < ab) || x >= abc;
-            u = (u << 1) | bu;
-            v = (v << 1) | bv;
-        }
-        src[e] = (int32_t)u;
-        dst[e] = (int32_t)v;
-    }
+    gen_rmat_range(scale, a, b, c, seed, 0, m, src, dst);
     return TRUST_OK;
 }
 

@@ -62,6 +62,10 @@ int preprocess_edges(const int32_t *src, const int32_t *dst, int64_t m, HostCsr 
 int batch_validate(const int64_t *beg, const int32_t *adj, int64_t V, const int64_t *vbase,
                    int64_t ngraphs);
 
+// Pairs [e0, e0 + count) of the seeded R-MAT stream of trust_gen_rmat.
+void gen_rmat_range(int scale, double a, double b, double c, uint64_t seed, int64_t e0,
+                    int64_t count, int32_t *src, int32_t *dst);
+
 // 2D hash partition tiles, see partition.cpp.
 int partition_tiles(const int64_t *beg, const int32_t *adj, int64_t V, int n,
                     std::vector<HostCsr> &tiles);

@@ -581,6 +581,29 @@ static int cmd_gen(int argc, char **argv)
     return 0;
 }
 
+// gen-ooc rmat <scale> <ef> <seed> <outdir> [workdir] [bucket_pairs]: the
+// R-MAT graph preprocessed out of core (trust_rmat_csr_ooc); follow with
+// "partition <n> <outdir>" for its tiles.
+static int cmd_gen_ooc(int argc, char **argv)
+{
+    if (argc < 5 || std::string(argv[0]) != "rmat")
+        return 2;
+    const int scale = atoi(argv[1]), ef = atoi(argv[2]);
+    const uint64_t seed = strtoull(argv[3], nullptr, 10);
+    const std::string outdir = argv[4];
+    const std::string work = argc > 5 ? argv[5] : outdir;
+    const long long bucket = argc > 6 ? atoll(argv[6]) : (1ll << 27);
+    if (scale < 1 || scale > 30 || ef < 1)
+        return 2;
+    const int rc = trust_rmat_csr_ooc(scale, (int64_t)ef << scale, 0.57, 0.19, 0.19, seed,
+                                      outdir.c_str(), work.c_str(), bucket);
+    if (rc)
+        return die(rc);
+    printf("%s: R-MAT scale %d, edge factor %d, seed %llu, preprocessed out of core\n",
+           outdir.c_str(), scale, ef, (unsigned long long)seed);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     int rc = 2;
@@ -598,6 +621,8 @@ int main(int argc, char **argv)
             rc = cmd_partition(argc - 2, argv + 2);
         else if (cmd == "gen")
             rc = cmd_gen(argc - 2, argv + 2);
+        else if (cmd == "gen-ooc")
+            rc = cmd_gen_ooc(argc - 2, argv + 2);
     }
     if (rc == 2)
         fprintf(stderr,
@@ -609,6 +634,7 @@ int main(int argc, char **argv)
                 "       trust_tool gen rmat <scale> <ef> <seed> <outdir>\n"
                 "       trust_tool gen chunglu <n> <m> <gamma> <maxdeg> <seed> <outdir>\n"
                 "       trust_tool gen er <n> <m> <seed> <outdir>\n"
-                "       trust_tool gen ba <n> <k> <seed> <outdir>\n");
+                "       trust_tool gen ba <n> <k> <seed> <outdir>\n"
+                "       trust_tool gen-ooc rmat <scale> <ef> <seed> <outdir> [workdir] [bucket_pairs]\n");
     return rc;
 }
