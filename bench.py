@@ -168,17 +168,42 @@ def run_steps(fn, warmup, steps, label, E):
     cnt_s = sum(r.count_seconds for r in res) / len(res)
     idx_s = sum(r.index_seconds for r in res) / len(res)
     plan_s = sum(r.plan_seconds for r in res) / len(res)
+    dense_s = sum(r.dense_seconds for r in res) / len(res)
+    core = (f", dense core {res[0].core_vertices} vertices: {res[0].dense_edges} edges in "
+            f"{dense_s * 1e3:.2f} ms of the count" if res[0].core_vertices else "")
     log(f"{label}: {res[0].triangles} triangles, {dev_s * 1e3:.2f} ms/step "
         f"(index {idx_s * 1e3:.2f} ms, plan {plan_s * 1e3:.2f} ms, count "
-        f"kernels {cnt_s * 1e3:.2f} ms), {E / dev_s / 1e9:.3f} GTEPS")
-    return dict(r=res[0], dev_s=dev_s, cnt_s=cnt_s, idx_s=idx_s,
+        f"kernels {cnt_s * 1e3:.2f} ms{core}), {E / dev_s / 1e9:.3f} GTEPS")
+    return dict(r=res[0], dev_s=dev_s, cnt_s=cnt_s, idx_s=idx_s, dense_s=dense_s,
                 wall_ms=(t_end - t_start) / steps * 1e3)
 
 
-def roofline(T, ctx, r0, cnt_s, kw, workload):
-    """Roofline object for the counting launches: algorithmic bytes / time,
-    against HBM peak, plus the measured load-path references (the same walk
-    with hashing replaced by a checksum, and a streaming copy)."""
+def core_row_bytes(nc):
+    """Bytes of bit rows one dense edge reads: the core's rows are nc bits
+    rounded up to S slices of a multiple of 4 words (core.hip)."""
+    S = 16 if nc > 16384 else 8
+    wd = max(4, ((nc + 32 * S - 1) // (32 * S) + 3) & ~3)
+    return 4 * wd * S
+
+
+def core_config(r, dense_s):
+    """The dense core's share of a count (None when it had none)."""
+    if not r.core_vertices:
+        return None
+    rb = core_row_bytes(r.core_vertices)
+    return {"vertices": r.core_vertices, "dense_edges": r.dense_edges,
+            "dense_probes": r.dense_probes, "W_probes_short_walk": r.probes + r.dense_probes,
+            "dense_ms": dense_s * 1e3, "row_bytes_per_edge": rb,
+            "bit_row_GBs": r.dense_edges * rb / dense_s / 1e9 if dense_s > 0 else None}
+
+
+def roofline(T, ctx, r0, cnt_s, kw, workload, dense_s=0.0):
+    """Roofline object for the hashed counting launches: algorithmic bytes /
+    time, against HBM peak, plus the measured load-path references (the same
+    walk with hashing replaced by a checksum, and a streaming copy).  The
+    dense core's launch (dense_s of the count's time, bound by the L2s that
+    hold its bit rows) is left out of this kernel's time."""
+    cnt_s = cnt_s - dense_s
     achieved = r0.alg_bytes / cnt_s / 1e9
     gkw = dict(kw, schedule=T.SCHED_DEGREE)
     ctx.gather(**gkw)
@@ -270,6 +295,7 @@ def bench_rmat(args, T):
                           "only_head": True, "index_ms": head["idx_s"] * 1e3,
                           "plan_ms": r0.plan_seconds * 1e3,
                           "count_kernel_ms": head["cnt_s"] * 1e3,
+                          "core": core_config(r0, head["dense_s"]),
                           "schedule_used": "learned" if r0.schedule_used == T.SCHED_LEARNED
                           else "degree",
                           "head_counts": args.warmup + args.steps,
@@ -359,6 +385,7 @@ def bench_rmat(args, T):
         "index_ms": head["idx_s"] * 1e3,
         "plan_ms": (dev_s - cnt_s - head["idx_s"]) * 1e3,
         "count_kernel_ms": cnt_s * 1e3,
+        "core": core_config(r0, head["dense_s"]),
         "gteps_index_kept": E / kept["dev_s"] / 1e9,
         "W_probes_reference_walk": ref_walk["r"].probes,
         "degree_split_gteps": E / deg_fresh["dev_s"] / 1e9,
@@ -375,7 +402,8 @@ def bench_rmat(args, T):
         "autotune": tune,
         "h2d_s": h2d, "preprocess_s": t2 - t1,
     }
-    line["roofline"] = roofline(T, ctx, r0, cnt_s, dict(head_kw, flags=0), workload)
+    line["roofline"] = roofline(T, ctx, r0, cnt_s, dict(head_kw, flags=0), workload,
+                                head["dense_s"])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(beg, adj, E, args.cpu_seconds, r0.triangles)
     ctx.close()
@@ -417,6 +445,7 @@ def bench_chung_lu(args, T):
         "parallelism": "single-gpu",
         "index_ms": head["idx_s"] * 1e3, "plan_ms": (dev_s - cnt_s - head["idx_s"]) * 1e3,
         "count_kernel_ms": cnt_s * 1e3,
+        "core": core_config(r0, head["dense_s"]),
         "gteps_index_kept": E / kept["dev_s"] / 1e9,
         "default_launch_gteps": E / deg["dev_s"] / 1e9,
         "reference_launch_gteps": E / ref_launch["dev_s"] / 1e9,
@@ -424,7 +453,7 @@ def bench_chung_lu(args, T):
         "speedup_vs_reference_launch": ref_launch["dev_s"] / dev_s,
         "autotune": tune,
     }
-    rl = roofline(T, ctx, r0, cnt_s, head_kw, workload)
+    rl = roofline(T, ctx, r0, cnt_s, head_kw, workload, head["dense_s"])
     # The graph (adj 66 MB, offsets 15 MB, start index 132 MB) is resident in
     # the 256 MB Infinity Cache, so the bound is the cache's bandwidth,
     # measured here with the same copy kernel on a cache-resident buffer.

@@ -94,7 +94,20 @@ typedef struct trust_params {
     int32_t lds_budget;    /* bytes of LDS per workgroup for the hub table */
     uint32_t flags;        /* TRUST_FLAG_* */
     int32_t device;        /* HIP device index */
-    int32_t reserved[5];
+    /* (after reserved) Dense core of the short-side walk (no counterpart in the reference,
+     * whose counts it leaves unchanged): the edges u -> v with d+(u) and
+     * d+(v) >= core_min_degree into a core vertex v whose out-list lies in
+     * the core are counted as popcount(B(u) & B(v)) over bit rows of the
+     * core, and the walk leaves them out.  The core is the core_vertices
+     * highest estimated degrees, closed under out-neighbours while it stays
+     * within 32768 vertices.  A per-graph index like the walk rows, made by
+     * the first count that uses it.  core_vertices: 0 automatic (16384 for
+     * single graphs of at least 2^23 edges counted by one logical rank, else
+     * none), -1 none, else a power of two in [256, 16384].
+     * core_min_degree: 0 automatic (384), else >= 2. */
+    int32_t reserved[3];   /* reserved[0]: a tile stream's device tile-cache slots (0: 2n+2) */
+    int32_t core_vertices;
+    int32_t core_min_degree;
 } trust_params;
 
 /* Result of one counting call. */
@@ -124,6 +137,10 @@ typedef struct trust_result {
                               context reuses them), included in seconds */
     uint64_t linear_rebuilds; /* hash tables rebuilt with linear probing after a cuckoo
                                  insertion failed (or was forced to, TRUST_FLAG_REBUILD_LINEAR) */
+    int64_t core_vertices;    /* vertices of the dense core this count used (0: none) */
+    uint64_t dense_edges;     /* oriented edges counted by the core's bit rows */
+    uint64_t dense_probes;    /* the short-side walk's probes of those edges (not in probes) */
+    double dense_seconds;     /* device time of the core's launch (part of count_seconds) */
 } trust_result;
 
 /* Library-allocated CSR (trust_csr_free releases it). */

@@ -53,7 +53,8 @@ class Params(C.Structure):
         ("threads", C.c_int32), ("blocks", C.c_int32), ("chunk", C.c_int32),
         ("rank", C.c_int32), ("total_process", C.c_int32), ("schedule", C.c_int32),
         ("hub_threshold", C.c_int32), ("wave_slots", C.c_int32), ("lds_budget", C.c_int32),
-        ("flags", C.c_uint32), ("device", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("flags", C.c_uint32), ("device", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("core_vertices", C.c_int32), ("core_min_degree", C.c_int32),
     ]
 
 
@@ -68,6 +69,8 @@ class Result(C.Structure):
         ("schedule_used", C.c_int32), ("walk_used", C.c_int32),
         ("index_seconds", C.c_double),
         ("linear_rebuilds", C.c_uint64),
+        ("core_vertices", C.c_int64), ("dense_edges", C.c_uint64), ("dense_probes", C.c_uint64),
+        ("dense_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -138,6 +138,31 @@ struct TileWalk {
     int64_t rows = 0, cols = 0, x_out = 0, x_in = 0;
 };
 
+// The dense core (core.hip): at most 32768 high-degree vertices (a seed of
+// K, closed under out-neighbours) with ids cid; the dense targets (core
+// vertices whose out-list lies in the core, d+ >= thr) with their bit rows,
+// sliced per XCD; the dense sources
+// (d+ >= thr, a repeat-free core part and a dense target) with their core
+// ids.  A per-graph index like the walk's, which leaves the edges from a
+// dense source to a dense target to the core's launch.
+struct CoreIndex {
+    DevBuf flag, cid, pos, coreV, bits, rows, pool, recs, bsrc, misc, hist;
+    long long nsrc_cap = 0;        // rows B(u) bsrc holds
+    int want_K = 0, want_thr = 0;  // the configuration the next build uses
+    bool want_auto = false;        // chosen automatically: only for heavy-tailed graphs
+    int K = 0, thr = 0;            // the configuration of the built index
+    int nC = 0;                    // core vertices
+    int Kb = 0, S = 0;             // bits of a row, slices of the rows
+    long long cap = 0;             // record slots
+    long long words = 0;           // 32-bit words of one V-bit bitmap
+    bool built = false;            // made for the resident graph (maybe empty)
+    bool active = false;           // built with dense targets: the walk leaves its edges out
+    uint64_t dense_probes = 0;     // the short walk's probes of the edges it took
+    uint64_t dense_edges = 0;
+    const uint32_t *dst_bits() const { return (const uint32_t *)bits.p; }
+    const uint32_t *src_bits() const { return (const uint32_t *)bits.p + words; }
+};
+
 struct Ctx {
     int device = 0;
     bool ready = false;
@@ -149,6 +174,8 @@ struct Ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // around the per-graph index work a count does before its plan
     hipEvent_t ev_idx[2] = {nullptr, nullptr};
+    // around the dense core's launch of a timed count
+    hipEvent_t ev_core[2] = {nullptr, nullptr};
     int num_cus = 0;
     size_t lds_max = 0;
     DevCsr g;
@@ -161,6 +188,7 @@ struct Ctx {
     bool side_stats_ready = false, side_built = false;
     bool side_rows_built = false;  // side_row: each walk entry's row (the learned features)
     WalkIndexState wi;
+    CoreIndex core;
     uint64_t side_w_ref = 0, side_w_short = 0;
     int64_t side_entries = -1;  // entries of the walk rows (-1: not read back yet)
     // learned scheduler state
@@ -216,7 +244,8 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
                   bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
-                  const SegView *seg = nullptr, bool plan_only = false);
+                  const SegView *seg = nullptr, bool plan_only = false,
+                  bool with_core = false);
 
 int count_resident(Ctx *ctx, const trust_params &p, int mode,
                    unsigned long long *cycles_out, unsigned long long *probes_out,
@@ -230,6 +259,8 @@ struct ResidentRun {
     bool side = false;           // short-side walk
     bool segmented = false;      // an int64 batch counted as int32 segments
     bool starts = false;         // the reference walk reads the start index
+    bool core = false;           // the walk rows leave the dense core's edges out
+    double dense_seconds = 0;    // device time of the core's launch (timed counts)
     bool index_ev = false;       // this count enqueued index work between ctx->ev_idx[0..1]
     double index_seconds = 0;    // device time of that work (after the count's sync)
 };
@@ -248,6 +279,14 @@ void release_walk_scratch(Ctx *ctx);
 // yields both walks' probe totals (synchronises); walk_index_build: the
 // rows from the counted graph (launches only).
 bool walk_index_binned(const Ctx *ctx);
+// Dense core (core.hip).  core_config: the core a count with p asks for
+// (K = 0: none); core_build: the index for ctx->core.want_K/want_thr
+// (synchronises once); core_enqueue: the counting launch, adding to
+// ctl->count; core_drop: forget the index.
+void core_config(const Ctx *ctx, const trust_params &p, int *K, int *thr, bool *automatic);
+int core_build(Ctx *ctx);
+int core_enqueue(Ctx *ctx, TcCtl *ctl, hipStream_t s);
+void core_drop(Ctx *ctx);
 int walk_index_count(Ctx *ctx);
 int walk_index_build(Ctx *ctx);
 // Builds the walk of a tile from its device CSR and the global out-degrees

@@ -269,7 +269,7 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
                   const trust_params &p, const LaunchPlan &L, TcCtl *ctl_dev, int mode,
                   unsigned long long *instr_cycles, unsigned long long *instr_probes,
                   bool preplanned, hipStream_t s, hipEvent_t ev_plan, hipEvent_t ev_count,
-                  const SegView *seg, bool plan_only)
+                  const SegView *seg, bool plan_only, bool with_core)
 {
     int rc = ctx->list.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
     if (rc)
@@ -310,6 +310,15 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
         HIPCHK(hipEventRecord(ev_plan, s));
     if (plan_only)
         return TRUST_OK;
+    // the dense core first: its bit rows have the L2s to themselves
+    if (with_core) {
+        if (ev_plan)
+            HIPCHK(hipEventRecord(ctx->ev_core[0], s));
+        if ((rc = core_enqueue(ctx, ctl_dev, s)))
+            return rc;
+        if (ev_plan)
+            HIPCHK(hipEventRecord(ctx->ev_core[1], s));
+    }
 
     // The large-LDS launch (big + spill tiers) and the main launch (hub +
     // wave tiers) share nothing but the result counters, so the big one runs
@@ -349,11 +358,11 @@ int enqueue_count(Ctx *ctx, const CsrViews &views, int64_t V, bool same_csr,
 template int enqueue_count<int32_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t, const SegView *, bool);
+                                    hipEvent_t, hipEvent_t, const SegView *, bool, bool);
 template int enqueue_count<int64_t>(Ctx *, const CsrViews &, int64_t, bool, const trust_params &,
                                     const LaunchPlan &, TcCtl *, int, unsigned long long *,
                                     unsigned long long *, bool, hipStream_t,
-                                    hipEvent_t, hipEvent_t, const SegView *, bool);
+                                    hipEvent_t, hipEvent_t, const SegView *, bool, bool);
 
 int check_params(const trust_params &p)
 {
@@ -376,6 +385,11 @@ int check_params(const trust_params &p)
         return fail(TRUST_ERR_ARG, "unknown flag bits 0x%x", p.flags & ~known);
     if ((p.flags & TRUST_FLAG_REFERENCE_WALK) && (p.flags & TRUST_FLAG_SHORT_WALK))
         return fail(TRUST_ERR_ARG, "REFERENCE_WALK and SHORT_WALK together");
+    const int K = p.core_vertices;
+    if (K < -1 || (K > 0 && (K < 256 || K > 16384 || (K & (K - 1)))))
+        return fail(TRUST_ERR_ARG, "core_vertices %d: -1, 0 or a power of two in [256, 16384]", K);
+    if (p.core_min_degree < 0 || p.core_min_degree == 1)
+        return fail(TRUST_ERR_ARG, "core_min_degree %d: 0 or >= 2", p.core_min_degree);
     return TRUST_OK;
 }
 
@@ -428,7 +442,26 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
         release_walk_scratch(ctx);
         ctx->wi.counted = false;
         binned = false;
+        // the atomic build walks every edge: no dense core
+        core_drop(ctx);
+        ctx->core.built = true;
         return TRUST_OK;
+    };
+    // The binned count pass leaves the dense core's edges out of the walk,
+    // so the core comes first; without HBM for it the walk keeps them.
+    auto count_pass = [&]() -> int {
+        if (!ctx->core.built) {
+            int r = core_build(ctx);
+            if (r == TRUST_ERR_OOM) {
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+                core_drop(ctx);
+                ctx->core.built = true;
+                r = TRUST_OK;
+            }
+            if (r)
+                return r;
+        }
+        return walk_index_count(ctx);
     };
     int rc;
     if (!force && !ctx->side_stats_ready) {
@@ -436,7 +469,7 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
             return rc;
         if (binned) {
             // the binned build's count pass yields the probe totals too
-            rc = walk_index_count(ctx);
+            rc = count_pass();
             if (rc == TRUST_ERR_OOM)
                 rc = leave_binned();
             if (rc)
@@ -468,7 +501,7 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
         const size_t ob = g.off64 ? sizeof(int64_t) : sizeof(int32_t);
         bool done = false;
         if (binned) {
-            rc = ctx->wi.counted ? TRUST_OK : walk_index_count(ctx);
+            rc = ctx->wi.counted ? TRUST_OK : count_pass();
             if (!rc)
                 rc = walk_index_build(ctx);
             if (rc == TRUST_ERR_OOM)
@@ -584,6 +617,7 @@ int ensure_walk_index(Ctx *ctx)
 // loaded graph without reloading it).
 static void drop_indexes(Ctx *ctx)
 {
+    core_drop(ctx);
     ctx->feat_walk = -1;
     ctx->side_stats_ready = ctx->side_built = ctx->side_rows_built = false;
     ctx->wi.counted = false;
@@ -609,13 +643,28 @@ int prepare_resident(Ctx *ctx, const trust_params &p, int mode, ResidentRun &R)
         HIPCHK(hipStreamSynchronize(ctx->stream));
         drop_indexes(ctx);
     }
+    // the walk rows leave out the edges of the core they were built with:
+    // a count that asks for another core makes the indexes again
+    int core_K, core_thr;
+    bool core_auto;
+    core_config(ctx, p, &core_K, &core_thr, &core_auto);
+    if (core_K != ctx->core.want_K || core_thr != ctx->core.want_thr ||
+        core_auto != ctx->core.want_auto) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        drop_indexes(ctx);
+        ctx->core.want_K = core_K;
+        ctx->core.want_thr = core_thr;
+        ctx->core.want_auto = core_auto;
+    }
     R.side = false;
+    R.core = false;
     R.index_ev = false;
     R.index_seconds = 0;
     if (!(p.flags & TRUST_FLAG_REFERENCE_WALK) && g.V > 0 && g.E > 0) {
         rc = prepare_side(ctx, (p.flags & TRUST_FLAG_SHORT_WALK) != 0, R);
         if (rc)
             return rc;
+        R.core = R.side && ctx->core.active;
     }
     if (mode == kModeInstr) {
         rc = ctx->instr_cycles.ensure(sizeof(unsigned long long) * (size_t)std::max<int64_t>(g.V, 1));
@@ -722,16 +771,19 @@ int enqueue_resident(Ctx *ctx, const trust_params &p, int mode, const ResidentRu
     hipEvent_t e1 = timed ? ctx->ev[1] : nullptr, e2 = timed ? ctx->ev[2] : nullptr;
     // the short-side walk probes long lists: six windows in flight per wave
     const int m = R.side && (mode == kModeCount || mode == kModeGather) ? mode | kProbeDeep : mode;
+    // the dense core's edges, left out of the walk, are counted beside it
+    // (the gather-only pass times the walk alone)
+    const bool core = R.core && (mode == kModeCount || mode == kModeInstr);
     if (g.off64)
         rc = enqueue_count<int64_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, m,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, R.learned,
-                                    ctx->stream, e1, e2);
+                                    ctx->stream, e1, e2, nullptr, false, core);
     else
         rc = enqueue_count<int32_t>(ctx, v, g.V, true, p, R.L, (TcCtl *)ctx->ctl.p, m,
                                     (unsigned long long *)ctx->instr_cycles.p,
                                     (unsigned long long *)ctx->instr_probes.p, R.learned,
-                                    ctx->stream, e1, e2);
+                                    ctx->stream, e1, e2, nullptr, false, core);
     return rc;
 }
 
@@ -777,6 +829,12 @@ int finish_resident(Ctx *ctx, ResidentRun &R, const TcCtl &h, double t_plan, dou
     r->schedule_used = R.learned ? TRUST_SCHED_LEARNED : TRUST_SCHED_DEGREE;
     r->walk_used = R.side ? 1 : 0;
     r->linear_rebuilds = h.rebuilds;
+    if (R.core) {
+        r->core_vertices = ctx->core.nC;
+        r->dense_edges = ctx->core.dense_edges;
+        r->dense_probes = ctx->core.dense_probes;
+        r->dense_seconds = R.dense_seconds;
+    }
     return TRUST_OK;
 }
 
@@ -809,6 +867,11 @@ int count_resident(Ctx *ctx, const trust_params &p, int mode,
     float t_plan = 0, t_count = 0;
     HIPCHK(hipEventElapsedTime(&t_plan, ctx->ev[0], ctx->ev[1]));
     HIPCHK(hipEventElapsedTime(&t_count, ctx->ev[1], ctx->ev[2]));
+    if (R.core && (mode == kModeCount || mode == kModeInstr)) {
+        float t_core = 0;
+        HIPCHK(hipEventElapsedTime(&t_core, ctx->ev_core[0], ctx->ev_core[1]));
+        R.dense_seconds = t_core * 1e-3;
+    }
     return finish_resident(ctx, R, h, t_plan * 1e-3, t_count * 1e-3, r);
 }
 
@@ -836,6 +899,8 @@ int ctx_init(Ctx *ctx, int device)
         HIPCHK(hipEventCreate(&ev));
     for (auto &ev : ctx->ev_idx)
         HIPCHK(hipEventCreate(&ev));
+    for (auto &ev : ctx->ev_core)
+        HIPCHK(hipEventCreate(&ev));
     ctx->ready = true;
     return TRUST_OK;
 }
@@ -854,6 +919,8 @@ Ctx::~Ctx()
     for (auto &ev : this->ev)
         (void)hipEventDestroy(ev);
     for (auto &ev : this->ev_idx)
+        (void)hipEventDestroy(ev);
+    for (auto &ev : this->ev_core)
         (void)hipEventDestroy(ev);
     (void)hipEventDestroy(ev_fork);
     (void)hipEventDestroy(ev_join);

@@ -111,8 +111,13 @@ struct WiPassArgs {
     int *M;                       // A: in-entries per bin and workgroup, bin-major
     unsigned long long *stats;    // A: probes of both walks (resident graph)
     // resident graph: d+ of each entry's vertex (edges of rows with d+ >= 2),
-    // written by A, read by B in place of a second pair of offset lookups
+    // written by A, read by B in place of a second pair of offset lookups;
+    // -1 marks an edge the dense core counts
     int32_t *dvE;
+    // the dense core's bitmaps (resident graph, null without a core): an
+    // edge u -> v with u a dense source and v a dense target is counted by
+    // bit rows and stays out of the walk
+    const uint32_t *core_src, *core_dst;
     const long long *O;           // B: each workgroup's first record in each bin
     const OffT *outBeg;           // B: each row's first out-entry in tmpOut
     typename WiFmt<OffT, kTile>::Out *tmpOut;
@@ -145,7 +150,7 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
     for (int k = 0; k < kWin; k++)
         mark[k * kWave + lane] = 0;
     __syncthreads();
-    unsigned long long w_ref = 0, w_short = 0;
+    unsigned long long w_ref = 0, w_short = 0, w_dense = 0, n_dense = 0;
     const long long step = (long long)G * kPassWaves * kWave;
     for (long long u0 = ((long long)blockIdx.x * kPassWaves + wave) * kWave; u0 < V; u0 += step) {
         const int n = (int)min((long long)kWave, V - u0);
@@ -158,6 +163,10 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
         OffT ob = 0;  // the row's first out-entry in the temporary
         if constexpr (kPass == 1)
             ob = lane < n ? a.outBeg[u0 + lane] : (OffT)0;
+        int src = 0;  // the row's vertex is a dense source (pass A's test)
+        if constexpr (kPass == 0 && !kTile)
+            if (a.core_src && lane < n)
+                src = (int)((a.core_src[(u0 + lane) >> 5] >> ((u0 + lane) & 31)) & 1u);
         const long long e0 = (long long)readlane(s, 0);
         const long long e1 = n == kWave ? (long long)readlane(s, 63) + readlane(d, 63)
                                         : (long long)readlane(s, n);
@@ -187,6 +196,10 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
 #pragma unroll
             for (int k = 0; k < kWin; k++) {
                 const int du = shfl_idx(rdeg, own[k]);
+                int srcu = 0;
+                if constexpr (kPass == 0 && !kTile)
+                    if (a.core_src)
+                        srcu = shfl_idx(src, own[k]);
                 OffT su = 0, obu = 0;  // shuffled while every lane is active
                 if constexpr (kPass == 1) {
                     if constexpr (!kTile)
@@ -208,6 +221,12 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
                         if constexpr (!kTile) {
                             w_ref += (unsigned long long)dv;
                             w_short += (unsigned long long)min(du, dv);
+                            if (srcu && ((a.core_dst[v[k] >> 5] >> (v[k] & 31)) & 1u)) {
+                                w_dense += (unsigned long long)min(du, dv);
+                                n_dense++;
+                                a.dvE[p + k * kWave + lane] = -1;
+                                continue;
+                            }
                             a.dvE[p + k * kWave + lane] = dv;
                         }
                         if (dv <= du)
@@ -215,6 +234,9 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
                         else
                             atomicAdd(bins + (v[k] >> kBinLg), (CurT)1);
                     } else {
+                        if constexpr (!kTile)
+                            if (dv < 0)
+                                continue;  // counted by the dense core
                         if (dv <= du) {
                             const int slot = atomicAdd(rowcnt + own[k], 1);
                             if constexpr (kTile)
@@ -246,9 +268,15 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
         if constexpr (!kTile) {
             w_ref = wave_sum(w_ref);
             w_short = wave_sum(w_short);
+            w_dense = wave_sum(w_dense);
+            n_dense = wave_sum(n_dense);
             if (lane == 0 && w_ref) {
                 atomicAdd(a.stats, w_ref);
                 atomicAdd(a.stats + 1, w_short);
+                if (n_dense) {
+                    atomicAdd(a.stats + 2, w_dense);
+                    atomicAdd(a.stats + 3, n_dense);
+                }
             }
         }
     }
@@ -578,11 +606,12 @@ bool walk_index_binned(const Ctx *ctx) { return ctx->g.V > 0 && walk_bins(ctx->g
 // Pass A and the scans over a CSR of `rows` rows whose in-entries target
 // [0, cols): kept and outBeg (rows + 1) on the device; the bins' first
 // records, the out-entry total and (resident graph) both walks' probe
-// totals on the host.  Synchronises.
+// totals, then the probes and edges the dense core took, on the host.
+// Synchronises.
 template <typename OffT, bool kTile>
 static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long rows, long long cols,
                     const int32_t *degR, const int32_t *degC, int *kept, OffT *outBeg,
-                    unsigned long long stats_out[2], int32_t *dvE)
+                    unsigned long long stats_out[4], int32_t *dvE)
 {
     WalkIndexState &w = ctx->wi;
     const int nbins = std::max(1, walk_bins(cols));
@@ -605,7 +634,7 @@ static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
         (rc = ctx->side_stats.ensure(4 * sizeof(unsigned long long))))
         return rc;
     unsigned long long *st = (unsigned long long *)ctx->side_stats.p;
-    HIPCHK(launch_fill_u32((uint32_t *)st, 0u, 4, ctx->stream));
+    HIPCHK(launch_fill_u32((uint32_t *)st, 0u, 8, ctx->stream));
     WiPassArgs<OffT, kTile> pa;
     memset(&pa, 0, sizeof(pa));
     pa.beg = beg;
@@ -618,6 +647,10 @@ static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
     pa.M = (int *)w.M.p;
     pa.stats = st;
     pa.dvE = dvE;
+    if (!kTile && ctx->core.active) {
+        pa.core_src = ctx->core.src_bits();
+        pa.core_dst = ctx->core.dst_bits();
+    }
     if (rows > 0) {
         hipLaunchKernelGGL((wi_pass_kernel<OffT, 0, kTile>), dim3(G), dim3(kPassThreads), lds,
                            ctx->stream, pa);
@@ -632,7 +665,7 @@ static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
     HIPCHK(hipGetLastError());
     HIPCHK(launch_scan<OffT>(kept, nullptr, rows, (long long *)w.bsum.p, outBeg, ctx->stream));
     w.bin_start_h.resize((size_t)nbins + 1);
-    unsigned long long h[2];
+    unsigned long long h[4];
     OffT out_total = 0;
     HIPCHK(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(w.bin_start_h.data(), w.bin_start.p, sizeof(long long) * (nbins + 1),
@@ -640,10 +673,9 @@ static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
     HIPCHK(hipMemcpyAsync(&out_total, outBeg + rows, sizeof(OffT), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (stats_out) {
-        stats_out[0] = h[0];
-        stats_out[1] = h[1];
-    }
+    if (stats_out)
+        for (int k = 0; k < 4; k++)
+            stats_out[k] = h[k];
     w.grid = G;
     w.nbins = nbins;
     w.out_total = (int64_t)out_total;
@@ -800,7 +832,7 @@ static int count_pass_t(Ctx *ctx)
         (rc = w.dv.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(g.E, 1))) ||
         (rc = ctx->side_cnt.ensure(2 * sizeof(int) * (size_t)V)))
         return rc;
-    unsigned long long h[2];
+    unsigned long long h[4];
     rc = wi_count<OffT, false>(ctx, (const OffT *)g.beg.p, (const int32_t *)g.adj.p, V, V, nullptr,
                                nullptr, (int *)ctx->side_cnt.p + V, (OffT *)w.out_beg.p, h,
                                (int32_t *)w.dv.p);
@@ -808,6 +840,8 @@ static int count_pass_t(Ctx *ctx)
         return rc;
     ctx->side_w_ref = h[0];
     ctx->side_w_short = h[1];
+    ctx->core.dense_probes = h[2];
+    ctx->core.dense_edges = h[3];
     ctx->side_stats_ready = true;
     w.counted = true;
     return TRUST_OK;
