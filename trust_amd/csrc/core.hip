@@ -553,9 +553,9 @@ static int core_build_t(Ctx *ctx)
     c.built = true;
     c.active = false;
     // The automatic core needs a heavy tail: its seed's smallest estimates at
-    // least 4t (R-MAT 26: about 12k; a Chung-Lu graph capped at degree 800
-    // has no such vertices, and its walk is short anyway)
-    if (take == 0 || (c.want_auto && bstar < core_bin_host(4ll * thr)))
+    // least 2t (R-MAT 26: about 12k; a Chung-Lu graph capped at degree 800
+    // has no such vertices, and takes the reference walk anyway)
+    if (take == 0 || (c.want_auto && bstar < core_bin_host(2ll * thr)))
         return TRUST_OK;
     hipLaunchKernelGGL(core_seed_kernel<OffT>, dim3(blocks), dim3(kCoreThreads), 0, st, beg, V,
                        (const int *)din, flag, bstar);
