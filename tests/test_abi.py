@@ -36,6 +36,32 @@ def test_library_exports_symbol(name):
     assert name in T.SIGNATURES  # the Python binding types every entry point
 
 
+def header_struct(name):
+    """(field, C type, array length) of `typedef struct name {...}` in trust.h."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct " + name + r" \{(.*?)\} " + name + ";", text, re.S).group(1)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(.+?)\s*\b(\w+)(\[(\d+)\])?$", decl)
+        ctype, field, n = m.group(1).replace("*", "").strip(), m.group(2), m.group(4)
+        out.append((field, ctype, int(n) if n else 1))
+    return out
+
+
+@pytest.mark.parametrize("name,py", [("trust_params", T.Params), ("trust_result", T.Result),
+                                     ("trust_arguments", T.Arguments)])
+def test_struct_layout_matches_header(name, py):
+    # the ctypes mirrors bind the same fields, in order, with the same sizes
+    sizes = {"int32_t": 4, "uint32_t": 4, "int64_t": 8, "uint64_t": 8, "double": 8}
+    fields = header_struct(name)
+    assert [f for f, _, _ in fields] == [f for f, _ in py._fields_]
+    for (field, ctype, n), (_, pytype) in zip(fields, py._fields_):
+        assert ctypes.sizeof(pytype) == sizes[ctype] * n, field
+
+
 def test_version_and_defaults():
     assert b"gfx950" in T.lib().trust_version()
     p = T.make_params()
