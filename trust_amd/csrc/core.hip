@@ -59,7 +59,7 @@ constexpr int kCoreSeedMax = 16384;  // K0
 constexpr int kCoreSeedMin = 256;
 constexpr int kCoreMax = 32768;      // |C| (int16 ids, at most 64 words per lane-row slice)
 constexpr int kCoreMaxSlices = 32;
-constexpr int kCoreRowsPerGrab = 256;  // rows per grab of the lists kernel's queue
+constexpr int kCoreGrabLg = 16;       // the lists kernel's grabs: 2^16 entries of adj
 constexpr int kCoreSteps = 6;        // closure steps
 // default t: R-MAT 26 fresh counts measured 253.6 ms at 256, 236.3 at 384,
 // 236.8 at 512; R-MAT 24 47.1, 47.5, 51.8 ms
@@ -272,6 +272,7 @@ core_lists_kernel(const OffT *__restrict__ beg, const int32_t *__restrict__ adj,
                   const int16_t *__restrict__ cid, const uint32_t *__restrict__ dense_cid, int Kb,
                   int thr, uint16_t *__restrict__ pool, DenseRec<OffT> *__restrict__ recs,
                   long long cap, uint32_t *__restrict__ bsrc, long long nsrc_cap,
+                  const long long *__restrict__ grab_row, long long ngrab,
                   unsigned *__restrict__ misc, uint32_t *__restrict__ src_bits)
 {
     constexpr int kWaves = kCoreThreads / kWave;
@@ -285,16 +286,18 @@ core_lists_kernel(const OffT *__restrict__ beg, const int32_t *__restrict__ adj,
     for (int i = lane; i < words; i += kWave)
         seen[i] = 0;
     __syncthreads();
-    // R-MAT's long rows crowd at low ids: groups of rows from a queue
+    // R-MAT's long rows crowd at low ids: the grabs are equal ranges of
+    // entries (the rows starting in 2^16 entries of adj), from a queue
     for (;;) {
         unsigned q = 0;
         if (lane == 0)
             q = atomicAdd(&misc[5], 1u);
-        const long long r0 = (long long)__builtin_amdgcn_readfirstlane(q) * kCoreRowsPerGrab;
-        if (r0 >= V)
+        q = __builtin_amdgcn_readfirstlane(q);
+        if ((long long)q >= ngrab)
             break;
-      for (long long u0 = r0; u0 < min(V, r0 + kCoreRowsPerGrab); u0 += kWave) {
-        const int n = (int)min((long long)kWave, V - u0);
+        const long long r0 = grab_row[q], r1 = grab_row[q + 1];
+      for (long long u0 = r0; u0 < r1; u0 += kWave) {
+        const int n = (int)min((long long)kWave, r1 - u0);
         const long long d = lane < n ? (long long)(beg[u0 + lane + 1] - beg[u0 + lane]) : 0;
         unsigned long long cand = __ballot(d >= thr);
         // a bsrc slot for each candidate row of the group
@@ -370,6 +373,32 @@ core_lists_kernel(const OffT *__restrict__ beg, const int32_t *__restrict__ adj,
         }
       }
     }
+}
+
+// grab_row[q] = the first row starting at or after entry q 2^kCoreGrabLg
+// (q <= ngrab; grab_row[ngrab] = V).
+template <typename OffT>
+__global__ void __launch_bounds__(kCoreThreads)
+core_grabs_kernel(const OffT *__restrict__ beg, long long V, long long ngrab,
+                  long long *__restrict__ grab_row)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > ngrab)
+        return;
+    if (q == ngrab) {
+        grab_row[q] = V;
+        return;
+    }
+    const long long at = q << kCoreGrabLg;
+    long long lo = 0, hi = V;  // the first u in [0, V] with beg[u] >= at
+    while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if ((long long)beg[mid] >= at)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    grab_row[q] = lo;
 }
 
 // Rows with d+ >= thr (an upper bound on the dense sources), into misc[4].
@@ -613,7 +642,8 @@ static int core_build_t(Ctx *ctx)
     if ((rc = c.rows.ensure((size_t)Kb * (size_t)Kb / 8)) ||
         (rc = c.pool.ensure(sizeof(uint16_t) * (size_t)E)) ||
         (rc = c.recs.ensure(sizeof(DenseRec<OffT>) * (size_t)c.cap)) ||
-        (rc = c.bsrc.ensure((size_t)c.nsrc_cap * (size_t)Kb / 8)))
+        (rc = c.bsrc.ensure((size_t)c.nsrc_cap * (size_t)Kb / 8)) ||
+        (rc = c.grabs.ensure(sizeof(long long) * (size_t)((E >> kCoreGrabLg) + 2))))
         return rc;
     HIPCHK(launch_scan<int32_t>(flag, nullptr, V, (long long *)ctx->side_bsum.p, (int32_t *)c.pos.p,
                                 st));
@@ -623,11 +653,15 @@ static int core_build_t(Ctx *ctx)
     hipLaunchKernelGGL(core_rows_kernel<OffT>, dim3((unsigned)nC), dim3(kCoreThreads), 0, st, beg,
                        adj, (const int16_t *)c.cid.p, (const int *)c.coreV.p, Kb, S, thr,
                        (uint32_t *)c.rows.p, dst_bits, dense_cid);
+    const long long ngrab = (E >> kCoreGrabLg) + 1;
+    hipLaunchKernelGGL(core_grabs_kernel<OffT>,
+                       dim3((unsigned)((ngrab + kCoreThreads) / kCoreThreads)), dim3(kCoreThreads),
+                       0, st, beg, V, ngrab, (long long *)c.grabs.p);
     hipLaunchKernelGGL(core_lists_kernel<OffT>, dim3((unsigned)std::max(xblocks, 1ll)),
                        dim3(kCoreThreads), 0, st, beg, adj, V, (const int16_t *)c.cid.p,
                        (const uint32_t *)dense_cid, Kb, thr, (uint16_t *)c.pool.p,
-                       (DenseRec<OffT> *)c.recs.p, c.cap, (uint32_t *)c.bsrc.p, c.nsrc_cap, misc,
-                       src_bits);
+                       (DenseRec<OffT> *)c.recs.p, c.cap, (uint32_t *)c.bsrc.p, c.nsrc_cap,
+                       (const long long *)c.grabs.p, ngrab, misc, src_bits);
     HIPCHK(hipGetLastError());
     c.active = true;
     return TRUST_OK;

@@ -146,7 +146,7 @@ struct TileWalk {
 // ids.  A per-graph index like the walk's, which leaves the edges from a
 // dense source to a dense target to the core's launch.
 struct CoreIndex {
-    DevBuf flag, cid, pos, coreV, bits, rows, pool, recs, bsrc, misc, hist;
+    DevBuf flag, cid, pos, coreV, bits, rows, pool, recs, bsrc, grabs, misc, hist;
     long long nsrc_cap = 0;        // rows B(u) bsrc holds
     int want_K = 0, want_thr = 0;  // the configuration the next build uses
     bool want_auto = false;        // chosen automatically: only for heavy-tailed graphs
