@@ -53,6 +53,38 @@ def test_oom_error_then_fallback_then_recovery(gpu, rmat18):
         T.set_device_memory_cap(0)
 
 
+def test_dense_core_without_room_is_left_out(gpu, rmat18):
+    # a core asked for on a graph whose core buffers (ids, target lists: 2
+    # bytes per entry) do not fit: the count goes on without it, exactly
+    beg, adj, want = rmat18
+    V, E = len(beg) - 1, len(adj)
+    kw = dict(flags=T.FLAG_SHORT_WALK | T.FLAG_FRESH_INDEX, core_vertices=1024,
+              core_min_degree=16)
+    try:
+        with T.Context(gpu) as c:
+            c.load(beg, adj)
+            full = c.count(**kw)
+            assert full.triangles == want and full.core_vertices > 0
+        with T.Context(gpu) as c:
+            c.load(beg, adj)
+            held = T.device_memory_held()
+            cap = held + 8 * E + 24 * V + (8 << 20)
+            print(f"[gpu] cap {cap} bytes: no room for the core", flush=True)
+            T.set_device_memory_cap(cap)
+            r = c.count(**kw)
+            assert r.triangles == want and r.walk_used == 1
+            assert r.core_vertices == 0 and r.dense_edges == 0
+            assert r.probes == oracle.probes_short(beg, adj)
+            assert T.device_memory_held() <= cap
+            # with room again, the next fresh count has its core
+            T.set_device_memory_cap(0)
+            r2 = c.count(**kw)
+            assert r2.triangles == want and r2.core_vertices == full.core_vertices
+            assert r2.probes + r2.dense_probes == oracle.probes_short(beg, adj)
+    finally:
+        T.set_device_memory_cap(0)
+
+
 def test_oom_in_batch_and_tiles_is_clean(gpu, rmat18):
     beg, adj, want = rmat18
     tiles = T.partition_csr(beg, adj, 2)

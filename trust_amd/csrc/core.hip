@@ -531,6 +531,15 @@ void core_drop(Ctx *ctx)
     c.dense_edges = c.dense_probes = 0;
 }
 
+void core_release(Ctx *ctx)
+{
+    CoreIndex &c = ctx->core;
+    core_drop(ctx);
+    for (DevBuf *b : {&c.flag, &c.cid, &c.pos, &c.coreV, &c.bits, &c.rows, &c.pool, &c.recs,
+                      &c.bsrc, &c.grabs, &c.misc, &c.hist})
+        b->release();
+}
+
 template <typename OffT>
 static int core_build_t(Ctx *ctx)
 {

@@ -287,6 +287,8 @@ void core_config(const Ctx *ctx, const trust_params &p, int *K, int *thr, bool *
 int core_build(Ctx *ctx);
 int core_enqueue(Ctx *ctx, TcCtl *ctl, hipStream_t s);
 void core_drop(Ctx *ctx);
+// core_drop and free the core's device memory (a synchronised stream)
+void core_release(Ctx *ctx);
 int walk_index_count(Ctx *ctx);
 int walk_index_build(Ctx *ctx);
 // Builds the walk of a tile from its device CSR and the global out-degrees

@@ -443,7 +443,7 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
         ctx->wi.counted = false;
         binned = false;
         // the atomic build walks every edge: no dense core
-        core_drop(ctx);
+        core_release(ctx);
         ctx->core.built = true;
         return TRUST_OK;
     };
@@ -454,7 +454,7 @@ static int prepare_side(Ctx *ctx, bool force, ResidentRun &R)
             int r = core_build(ctx);
             if (r == TRUST_ERR_OOM) {
                 HIPCHK(hipStreamSynchronize(ctx->stream));
-                core_drop(ctx);
+                core_release(ctx);
                 ctx->core.built = true;
                 r = TRUST_OK;
             }
