@@ -57,9 +57,9 @@ namespace trust {
 
 constexpr int kCoreSeedMax = 16384;  // K0
 constexpr int kCoreSeedMin = 256;
-constexpr int kCoreMax = 32768;      // |C| (int16 ids, at most 64 words per lane-row slice)
+constexpr int kCoreMax = 32768;      // |C| (int16 ids; at most 32 slices of 32 words)
 constexpr int kCoreMaxSlices = 32;
-constexpr int kCoreGrabLg = 16;       // the lists kernel's grabs: 2^16 entries of adj
+constexpr int kCoreGrabLg = 16;      // the lists kernel's grabs: 2^16 entries of adj
 constexpr int kCoreSteps = 6;        // closure steps
 // default t: R-MAT 26 fresh counts measured 253.6 ms at 256, 236.3 at 384,
 // 236.8 at 512; R-MAT 24 47.1, 47.5, 51.8 ms
@@ -71,7 +71,7 @@ constexpr int kCoreThreads = 256;
 constexpr int kCoreQueueStride = 32; // one 128-byte line per slice queue
 // misc words: [0] items of long lists (front of recs), [1] items of short
 // lists (back of recs), [2] closure additions, [3] bsrc slots taken, [4] rows
-// with d+ >= t, [5] the lists kernel's row queue, then the slice queues
+// with d+ >= t, [5] the lists kernel's grab queue, then the slice queues
 constexpr int kCoreMiscWords = 64 + kCoreMaxSlices * kCoreQueueStride;
 
 // A work item: dense targets pool[lo + t0, lo + t1) of the source whose
