@@ -15,9 +15,10 @@
 // 4-byte ids.  The walk index leaves these edges out (walk_index.hip, pass
 // A) and one launch counts them.
 //
-// Choosing C.  Orientation runs from lower to higher degree, so the set of
-// vertices above any degree is closed under out-neighbours and all of its
-// members with d+ >= t are dense targets.  Exact in-degrees would cost a
+// Choosing C.  Orientation runs from lower to higher degree (the degree the
+// preprocessing ordered by), so the set of vertices above any such degree is
+// closed under out-neighbours and all of its members with d+ >= t are dense
+// targets.  Exact in-degrees would cost a
 // global atomic per edge; an estimate (d+ plus 64x the in-count over every
 // 64th entry of adj) picks the seed, the K0 highest estimates in a 64-bins-
 // per-octave histogram, and the seed is then closed: each step adds the
