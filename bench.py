@@ -186,15 +186,19 @@ def core_row_bytes(nc):
     return 4 * wd * S
 
 
-def core_config(r, dense_s):
-    """The dense core's share of a count (None when it had none)."""
+def core_config(T, r, dense_s):
+    """The dense core's share of a count (None when it had none), with the
+    L2s' measured read rate its bit rows are read at."""
     if not r.core_vertices:
         return None
     rb = core_row_bytes(r.core_vertices)
+    gbs = r.dense_edges * rb / dense_s / 1e9 if dense_s > 0 else None
+    l2 = T.membench_read(2 << 20) / 1e9
+    log(f"dense core: {gbs:.0f} GB/s of bit rows; an L2-resident 2 MiB read: {l2:.0f} GB/s")
     return {"vertices": r.core_vertices, "dense_edges": r.dense_edges,
             "dense_probes": r.dense_probes, "W_probes_short_walk": r.probes + r.dense_probes,
-            "dense_ms": dense_s * 1e3, "row_bytes_per_edge": rb,
-            "bit_row_GBs": r.dense_edges * rb / dense_s / 1e9 if dense_s > 0 else None}
+            "dense_ms": dense_s * 1e3, "row_bytes_per_edge": rb, "bit_row_GBs": gbs,
+            "l2_read_GBs": l2, "frac_of_l2_read": gbs / l2 if gbs else None}
 
 
 def roofline(T, ctx, r0, cnt_s, kw, workload, dense_s=0.0):
@@ -295,7 +299,7 @@ def bench_rmat(args, T):
                           "only_head": True, "index_ms": head["idx_s"] * 1e3,
                           "plan_ms": r0.plan_seconds * 1e3,
                           "count_kernel_ms": head["cnt_s"] * 1e3,
-                          "core": core_config(r0, head["dense_s"]),
+                          "core": core_config(T, r0, head["dense_s"]),
                           "schedule_used": "learned" if r0.schedule_used == T.SCHED_LEARNED
                           else "degree",
                           "head_counts": args.warmup + args.steps,
@@ -385,7 +389,7 @@ def bench_rmat(args, T):
         "index_ms": head["idx_s"] * 1e3,
         "plan_ms": (dev_s - cnt_s - head["idx_s"]) * 1e3,
         "count_kernel_ms": cnt_s * 1e3,
-        "core": core_config(r0, head["dense_s"]),
+        "core": core_config(T, r0, head["dense_s"]),
         "gteps_index_kept": E / kept["dev_s"] / 1e9,
         "W_probes_reference_walk": ref_walk["r"].probes,
         "degree_split_gteps": E / deg_fresh["dev_s"] / 1e9,
@@ -445,7 +449,7 @@ def bench_chung_lu(args, T):
         "parallelism": "single-gpu",
         "index_ms": head["idx_s"] * 1e3, "plan_ms": (dev_s - cnt_s - head["idx_s"]) * 1e3,
         "count_kernel_ms": cnt_s * 1e3,
-        "core": core_config(r0, head["dense_s"]),
+        "core": core_config(T, r0, head["dense_s"]),
         "gteps_index_kept": E / kept["dev_s"] / 1e9,
         "default_launch_gteps": E / deg["dev_s"] / 1e9,
         "reference_launch_gteps": E / ref_launch["dev_s"] / 1e9,

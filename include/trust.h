@@ -338,6 +338,10 @@ int trust_ctx_gather(trust_ctx *ctx, const trust_params *p, trust_result *r);
 /* Device-to-device streaming copy of `bytes`, `reps` times; seconds per copy
  * (the known-traffic kernel that calibrates HBM byte counters). */
 int trust_membench_copy(int device, int64_t bytes, int reps, double *seconds_per_copy);
+/* Read rate (bytes per second) of 8 workgroups per CU each reading a buffer
+ * of `bytes` `reps` times: a buffer that fits the L2 (<= 4 MB) measures the
+ * L2s' read rate, the reference for the dense core's bit-row reads. */
+int trust_membench_read(int device, int64_t bytes, int reps, double *bytes_per_second);
 
 /* ---- launch autotuner (no counterpart in the reference) --------------- */
 /*

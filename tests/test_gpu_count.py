@@ -350,3 +350,12 @@ def test_start_index_default_walk_graphs(gpu):
             assert r.walk_used == 0
             assert r.triangles == oracle.count_hash(beg, adj)
             assert r.triangles == c.count(flags=T.FLAG_OFFSET_LOOKUP).triangles
+
+
+def test_membench_read_rate(gpu):
+    # the L2 read reference: a 2 MiB buffer every workgroup re-reads stays in
+    # the L2s, whose read rate is well above HBM's 8 TB/s
+    l2 = T.membench_read(2 << 20)
+    hbm = T.membench_read(1 << 30, reps=1)
+    print(f"[gpu] read rate: 2 MiB {l2 / 1e12:.1f} TB/s, 1 GiB {hbm / 1e12:.1f} TB/s", flush=True)
+    assert l2 > 10e12 and l2 > hbm

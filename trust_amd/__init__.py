@@ -150,6 +150,7 @@ SIGNATURES = {
                                         C.POINTER(C.c_double), C.POINTER(Result)]),
     "trust_ctx_batch_segments": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "trust_membench_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+    "trust_membench_read": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "trust_csr_load": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "trust_csr_save": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "trust_csr_validate": (C.c_int, [_P64, _P32, C.c_int64, C.c_int64]),
@@ -209,6 +210,13 @@ def membench_copy(nbytes: int, reps: int = 5, device: int = 0) -> float:
     """Seconds per device-to-device streaming copy of nbytes."""
     t = C.c_double(0)
     _check(lib().trust_membench_copy(device, nbytes, reps, C.byref(t)))
+    return t.value
+
+
+def membench_read(nbytes: int, reps: int = 8, device: int = 0) -> float:
+    """Bytes per second of every workgroup (8 per CU) reading an nbytes buffer."""
+    t = C.c_double(0)
+    _check(lib().trust_membench_read(device, nbytes, reps, C.byref(t)))
     return t.value
 
 

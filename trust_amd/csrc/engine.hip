@@ -1088,6 +1088,34 @@ extern "C" int trust_ctx_gather(trust_ctx *c, const trust_params *p, trust_resul
     return count_resident(ctx, *p, kModeGather, nullptr, nullptr, r);
 }
 
+extern "C" int trust_membench_read(int device, int64_t bytes, int reps, double *bytes_per_second)
+{
+    if (bytes < 16 || reps < 1 || !bytes_per_second)
+        return fail(TRUST_ERR_ARG, "bad arguments");
+    Ctx ctx;
+    int rc = ctx_init(&ctx, device);
+    if (rc)
+        return rc;
+    const long long n = bytes / 16;
+    const int blocks = ctx.num_cus * 8;
+    DevBuf a, sink;
+    rc = a.ensure((size_t)n * 16);
+    if (!rc)
+        rc = sink.ensure(sizeof(unsigned) * (size_t)blocks);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemsetAsync(a.p, 1, (size_t)n * 16, ctx.stream));
+    HIPCHK(launch_read((const int4 *)a.p, n, 1, blocks, (unsigned *)sink.p, ctx.stream));  // warm-up
+    HIPCHK(hipEventRecord(ctx.ev[0], ctx.stream));
+    HIPCHK(launch_read((const int4 *)a.p, n, reps, blocks, (unsigned *)sink.p, ctx.stream));
+    HIPCHK(hipEventRecord(ctx.ev[1], ctx.stream));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx.ev[0], ctx.ev[1]));
+    *bytes_per_second = (double)blocks * reps * (double)n * 16.0 / (ms * 1e-3);
+    return TRUST_OK;
+}
+
 extern "C" int trust_membench_copy(int device, int64_t bytes, int reps, double *seconds_per_copy)
 {
     if (bytes < 16 || reps < 1 || !seconds_per_copy)

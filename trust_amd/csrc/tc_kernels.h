@@ -141,6 +141,9 @@ hipError_t launch_count(const TcArgs<OffT> &a, int blocks, int threads, size_t l
 template <typename OffT>
 hipError_t count_occupancy(int threads, size_t lds_bytes, int mode, bool big, int *blocks_per_cu);
 hipError_t launch_copy(const int4 *src, int4 *dst, long long n, int blocks, hipStream_t stream);
+// every one of `blocks` workgroups reads src[0, n) `reps` times (sink: blocks words)
+hipError_t launch_read(const int4 *src, long long n, int reps, int blocks, unsigned *sink,
+                       hipStream_t stream);
 hipError_t launch_offsets_to_i32(const int64_t *src, int32_t *dst, long long n,
                                  hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, long long n, hipStream_t stream);

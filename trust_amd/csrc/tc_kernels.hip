@@ -1233,6 +1233,33 @@ hipError_t launch_copy(const int4 *src, int4 *dst, long long n, int blocks, hipS
     return hipGetLastError();
 }
 
+// Every workgroup reads the whole buffer `reps` times from its own starting
+// point: a buffer that fits the L2 measures the L2s' read rate (the dense
+// core's bit rows are read from there).
+__global__ void __launch_bounds__(256) read_kernel(const int4 *__restrict__ src, long long n,
+                                                   int reps, unsigned *__restrict__ sink)
+{
+    unsigned acc = 0;
+    const long long start = ((long long)blockIdx.x * 977) % n;
+    for (int r = 0; r < reps; r++)
+        for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+            long long j = i + start;
+            if (j >= n)
+                j -= n;
+            const int4 x = src[j];
+            acc += (unsigned)(x.x ^ x.y ^ x.z ^ x.w);
+        }
+    if (acc == 0x9e3779b9u)  // keeps the loads; practically never stores
+        sink[blockIdx.x] = acc;
+}
+
+hipError_t launch_read(const int4 *src, long long n, int reps, int blocks, unsigned *sink,
+                       hipStream_t stream)
+{
+    hipLaunchKernelGGL(read_kernel, dim3(blocks), dim3(256), 0, stream, src, n, reps, sink);
+    return hipGetLastError();
+}
+
 __global__ void fill_u32_kernel(uint32_t *dst, uint32_t value, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
