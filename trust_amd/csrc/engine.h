@@ -120,7 +120,7 @@ struct WiQueue {
 
 // Scratch and host-side state of the binned walk-index build.
 struct WalkIndexState {
-    DevBuf M, O, bin_start, bsum, out_beg, slices, chunks, recs, rel, sbase, tmp_out, dv;
+    DevBuf M, O, bin_start, bsum, out_beg, slices, chunks, recs, rel, sbase, tmp_out, dv, dtab;
     std::vector<long long> bin_start_h;  // first record of each bin (nbins + 1)
     std::vector<WiSlice> slices_h;
     std::vector<WiChunk> chunks_h;  // host copies stay alive until their copies run

@@ -557,6 +557,7 @@ void release_walk_scratch(Ctx *ctx)
     w.rel.release();
     w.sbase.release();
     w.tmp_out.release();
+    w.dtab.release();
 }
 
 int ensure_walk_rows(Ctx *ctx)

@@ -118,6 +118,10 @@ struct WiPassArgs {
     // edge u -> v with u a dense source and v a dense target is counted by
     // bit rows and stays out of the walk
     const uint32_t *core_src, *core_dst;
+    // resident graph, when every d+ < 2^15: d+(v) | (v a dense target) << 15,
+    // the one 2-byte gather pass A makes per entry (null: the offsets and the
+    // target map instead)
+    const uint16_t *dtab;
     const long long *O;           // B: each workgroup's first record in each bin
     const OffT *outBeg;           // B: each row's first out-entry in tmpOut
     typename WiFmt<OffT, kTile>::Out *tmpOut;
@@ -208,12 +212,18 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
                 }
                 if (v[k] >= 0 && du >= 2) {
                     OffT bv = 0;
-                    int dv;
+                    int dv, dstk = 0;
                     if constexpr (kTile) {
                         dv = a.degC[v[k]];
                     } else if constexpr (kPass == 0) {
-                        bv = beg[v[k]];
-                        dv = (int)(beg[v[k] + 1] - bv);
+                        if (a.dtab) {
+                            const unsigned x = a.dtab[v[k]];
+                            dv = (int)(x & 0x7fffu);
+                            dstk = (int)(x >> 15);
+                        } else {
+                            bv = beg[v[k]];
+                            dv = (int)(beg[v[k] + 1] - bv);
+                        }
                     } else {
                         dv = dvk[k];  // an out-entry looks up its list start below
                     }
@@ -221,7 +231,8 @@ __global__ void __launch_bounds__(kPassThreads) wi_pass_kernel(WiPassArgs<OffT, 
                         if constexpr (!kTile) {
                             w_ref += (unsigned long long)dv;
                             w_short += (unsigned long long)min(du, dv);
-                            if (srcu && ((a.core_dst[v[k] >> 5] >> (v[k] & 31)) & 1u)) {
+                            if (srcu && (a.dtab ? dstk != 0
+                                                : ((a.core_dst[v[k] >> 5] >> (v[k] & 31)) & 1u))) {
                                 w_dense += (unsigned long long)min(du, dv);
                                 n_dense++;
                                 a.dvE[p + k * kWave + lane] = -1;
@@ -598,6 +609,19 @@ __global__ void wi_bin_starts_kernel(const long long *__restrict__ O, int G, int
         out[b] = O[(long long)b * G];  // O[nbins * G] is the total
 }
 
+template <typename OffT>
+__global__ void __launch_bounds__(256) wi_dtab_kernel(const OffT *__restrict__ beg, long long V,
+                                                      const uint32_t *__restrict__ core_dst,
+                                                      uint16_t *__restrict__ dtab)
+{
+    for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < V;
+         x += (long long)gridDim.x * blockDim.x) {
+        const unsigned d = (unsigned)(beg[x + 1] - beg[x]);
+        const unsigned t = core_dst ? (core_dst[x >> 5] >> (x & 31)) & 1u : 0u;
+        dtab[x] = (uint16_t)(d | (t << 15));
+    }
+}
+
 // ------------------------------------------------------------------ host
 int walk_bins(long long V) { return (int)((V + kBinSize - 1) / kBinSize); }
 
@@ -650,6 +674,15 @@ static int wi_count(Ctx *ctx, const OffT *beg, const int32_t *adj, long long row
     if (!kTile && ctx->core.active) {
         pa.core_src = ctx->core.src_bits();
         pa.core_dst = ctx->core.dst_bits();
+    }
+    if (!kTile && rows > 0 && ctx->g.max_deg < (1 << 15)) {
+        if ((rc = w.dtab.ensure(sizeof(uint16_t) * (size_t)rows)))
+            return rc;
+        const long long blocks = std::min<long long>((rows + 255) / 256, 32ll * ctx->num_cus);
+        hipLaunchKernelGGL(wi_dtab_kernel<OffT>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                           beg, rows, pa.core_dst, (uint16_t *)w.dtab.p);
+        HIPCHK(hipGetLastError());
+        pa.dtab = (const uint16_t *)w.dtab.p;
     }
     if (rows > 0) {
         hipLaunchKernelGGL((wi_pass_kernel<OffT, 0, kTile>), dim3(G), dim3(kPassThreads), lds,
