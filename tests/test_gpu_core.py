@@ -179,3 +179,24 @@ def test_core_params_rejected(ctx16):
                dict(core_vertices=-2), dict(core_min_degree=1), dict(core_min_degree=-1)):
         with pytest.raises(T.TrustError):
             ctx16.count(**kw)
+
+
+def test_core_closure_past_its_limit_keeps_the_seed(gpu):
+    # vertex 0 points at 40000 vertices, each at the next: the closure of a
+    # seed holding 0 passes 32768 vertices, so the seed alone is the core
+    # (whose lists leave it: no dense targets), and the count stays exact:
+    # one triangle (0, i, i + 1) per i
+    n = 40000
+    beg = np.zeros(n + 2, np.int64)
+    beg[1] = n
+    beg[2:] = n + np.minimum(np.arange(1, n + 1), n - 1)
+    adj = np.concatenate([np.arange(1, n + 1), np.arange(2, n + 1)]).astype(np.int32)
+    assert len(adj) == beg[-1]
+    want = oracle.count_hash(beg, adj)
+    assert want == n - 1
+    print("[gpu] core closure past 32768 vertices", flush=True)
+    r = T.count_csr(beg, adj, device=gpu, flags=T.FLAG_SHORT_WALK, core_vertices=256,
+                    core_min_degree=2)
+    assert r.triangles == want
+    assert r.core_vertices <= 256 and r.dense_edges == 0
+    assert r.probes + r.dense_probes == oracle.probes_short(beg, adj)
