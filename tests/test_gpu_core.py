@@ -200,3 +200,30 @@ def test_core_closure_past_its_limit_keeps_the_seed(gpu):
     assert r.triangles == want
     assert r.core_vertices <= 256 and r.dense_edges == 0
     assert r.probes + r.dense_probes == oracle.probes_short(beg, adj)
+
+
+def test_core_closure_retried_from_a_halved_seed(gpu):
+    # 200 vertices (d+ 1001, no in-edges) point at 1001 targets with lists of
+    # two inside them; 100 vertices (d+ 600) point at 60000 others.  A seed of
+    # 512 holds both groups and closes past 32768 vertices; halved to 256 it
+    # holds the first group only, which closes to 1201 vertices (the seed
+    # alone, 457 vertices, would be the core without the retry).  Dense
+    # targets: the 999 targets with two entries; dense edges: 200 x 999 from
+    # the first group and the 998 + 997 between targets
+    K, T_, B, U = 200, 1001, 100, 60000
+    t0, b0, u0 = K, K + T_, K + T_ + B
+    rows = [list(range(t0, t0 + T_)) for _ in range(K)]
+    rows += [[x for x in (t + 1, t + 2) if x < t0 + T_] for t in range(t0, t0 + T_)]
+    rows += [list(range(u0 + 600 * i, u0 + 600 * (i + 1))) for i in range(B)]
+    rows += [[] for _ in range(U)]
+    beg = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    adj = np.array([x for r in rows for x in r], np.int32)
+    want = oracle.count_hash(beg, adj)
+    assert want == K * (2 * (T_ - 2) + 1) + (T_ - 2)
+    print("[gpu] core closure retried from a halved seed", flush=True)
+    r = T.count_csr(beg, adj, device=gpu, flags=T.FLAG_SHORT_WALK, core_vertices=512,
+                    core_min_degree=2)
+    assert r.triangles == want
+    assert r.core_vertices == K + T_
+    assert r.dense_edges == K * (T_ - 2) + (T_ - 3) + (T_ - 4)
+    assert r.probes + r.dense_probes == oracle.probes_short(beg, adj)

@@ -23,7 +23,8 @@
 // 64th entry of adj) picks the seed, the K0 highest estimates in a 64-bins-
 // per-octave histogram, and the seed is then closed: each step adds the
 // out-neighbours of the newest members, until nothing is added (at most
-// kCoreSteps steps and 32768 vertices; past either the seed alone is kept).
+// kCoreSteps steps; a closure past 32768 vertices is retried from a halved
+// seed, and below 256 the seed alone is kept).
 // R-MAT's degrees come in classes (a vertex's expected degree depends on the
 // bit pattern of its id): at scale 26 the 16k highest estimates close to
 // 17.9k vertices whose edges carry 52% of the short-side walk's probes
@@ -581,16 +582,21 @@ static int core_build_t(Ctx *ctx)
     HIPCHK(hipMemcpyAsync(h.data(), c.hist.p, sizeof(unsigned) * kCoreBins,
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    // the seed: the top bins, at most K0 vertices, estimates of at least thr
-    long long take = 0;
-    int bstar = kCoreBins;
+    // the seed: the top bins, at most k0 vertices, estimates of at least thr
     const int bmin = core_bin_host(thr);
-    for (int b = kCoreBins - 1; b >= bmin; b--) {
-        if (take + h[(size_t)b] > (unsigned long long)K0)
-            break;
-        take += h[(size_t)b];
-        bstar = b;
-    }
+    auto seed_cut = [&](long long k0, long long *take) {
+        int bstar = kCoreBins;
+        *take = 0;
+        for (int b = kCoreBins - 1; b >= bmin; b--) {
+            if (*take + h[(size_t)b] > (unsigned long long)k0)
+                break;
+            *take += h[(size_t)b];
+            bstar = b;
+        }
+        return bstar;
+    };
+    long long take = 0;
+    int bstar = seed_cut(K0, &take);
     c.K = K0;
     c.thr = thr;
     c.built = true;
@@ -600,33 +606,45 @@ static int core_build_t(Ctx *ctx)
     // has no such vertices, and takes the reference walk anyway)
     if (take == 0 || (c.want_auto && bstar < core_bin_host(2ll * thr)))
         return TRUST_OK;
-    hipLaunchKernelGGL(core_seed_kernel<OffT>, dim3(blocks), dim3(kCoreThreads), 0, st, beg, V,
-                       (const int *)din, flag, bstar);
-    HIPCHK(hipGetLastError());
-    // the closure, while it fits
-    long long members = take;
+    // The closure, while it fits; a seed whose closure passes kCoreMax
+    // vertices is halved and closed again (down to kCoreSeedMin), and past
+    // that the seed alone is the core (its targets are whichever lists lie
+    // inside it).
     const long long xblocks =
         std::min<long long>((V + kCoreThreads - 1) / kCoreThreads, 8ll * ctx->num_cus);
-    bool closed = false;
-    for (int step = 0; step < kCoreSteps && !closed; step++) {
-        HIPCHK(launch_fill_u32(misc + 2, 0u, 1, st));
-        hipLaunchKernelGGL(core_expand_kernel<OffT>, dim3((unsigned)std::max(xblocks, 1ll)),
-                           dim3(kCoreThreads), 0, st, beg, adj, V, flag, misc + 2);
-        hipLaunchKernelGGL(core_promote_kernel, dim3(blocks), dim3(kCoreThreads), 0, st, V, flag,
-                           0);
-        HIPCHK(hipGetLastError());
-        unsigned added = 0;
-        HIPCHK(hipMemcpyAsync(&added, misc + 2, sizeof(added), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        members += added;
-        closed = added == 0;
-        if (members > kCoreMax)
-            break;
-    }
-    if (members > kCoreMax) {  // too large: the seed alone
+    long long members = take;
+    for (long long k0 = K0;;) {
         hipLaunchKernelGGL(core_seed_kernel<OffT>, dim3(blocks), dim3(kCoreThreads), 0, st, beg,
                            V, (const int *)din, flag, bstar);
+        HIPCHK(hipGetLastError());
         members = take;
+        bool closed = false;
+        for (int step = 0; step < kCoreSteps && !closed && members <= kCoreMax; step++) {
+            HIPCHK(launch_fill_u32(misc + 2, 0u, 1, st));
+            hipLaunchKernelGGL(core_expand_kernel<OffT>, dim3((unsigned)std::max(xblocks, 1ll)),
+                               dim3(kCoreThreads), 0, st, beg, adj, V, flag, misc + 2);
+            hipLaunchKernelGGL(core_promote_kernel, dim3(blocks), dim3(kCoreThreads), 0, st, V,
+                               flag, 0);
+            HIPCHK(hipGetLastError());
+            unsigned added = 0;
+            HIPCHK(hipMemcpyAsync(&added, misc + 2, sizeof(added), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            members += added;
+            closed = added == 0;
+        }
+        if (members <= kCoreMax)
+            break;
+        long long t2 = 0;
+        const int b2 = k0 / 2 >= kCoreSeedMin ? seed_cut(k0 / 2, &t2) : kCoreBins;
+        if (t2 == 0) {  // no smaller seed: the seed alone
+            hipLaunchKernelGGL(core_seed_kernel<OffT>, dim3(blocks), dim3(kCoreThreads), 0, st,
+                               beg, V, (const int *)din, flag, bstar);
+            members = take;
+            break;
+        }
+        k0 /= 2;
+        bstar = b2;
+        take = t2;
     }
     hipLaunchKernelGGL(core_promote_kernel, dim3(blocks), dim3(kCoreThreads), 0, st, V, flag, 1);
     HIPCHK(hipGetLastError());
