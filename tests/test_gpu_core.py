@@ -207,7 +207,7 @@ def test_core_closure_retried_from_a_halved_seed(gpu):
     # two inside them; 100 vertices (d+ 600) point at 60000 others.  A seed of
     # 512 holds both groups and closes past 32768 vertices; halved to 256 it
     # holds the first group only, which closes to 1201 vertices (the seed
-    # alone, 457 vertices, would be the core without the retry).  Dense
+    # alone would be the core without the retry).  Dense
     # targets: the 999 targets with two entries; dense edges: 200 x 999 from
     # the first group and the 998 + 997 between targets
     K, T_, B, U = 200, 1001, 100, 60000

@@ -19,8 +19,8 @@
 // preprocessing ordered by), so the set of vertices above any such degree is
 // closed under out-neighbours and all of its members with d+ >= t are dense
 // targets.  Exact in-degrees would cost a
-// global atomic per edge; an estimate (d+ plus 64x the in-count over every
-// 64th entry of adj) picks the seed, the K0 highest estimates in a 64-bins-
+// global atomic per edge; an estimate (d+ plus 128x the in-count over every
+// 128th entry of adj) picks the seed, the K0 highest estimates in a 64-bins-
 // per-octave histogram, and the seed is then closed: each step adds the
 // out-neighbours of the newest members, until nothing is added (at most
 // kCoreSteps steps; a closure past 32768 vertices is retried from a halved
@@ -66,7 +66,7 @@ constexpr int kCoreSteps = 6;        // closure steps
 // default t: R-MAT 26 fresh counts measured 253.6 ms at 256, 236.3 at 384,
 // 236.8 at 512; R-MAT 24 47.1, 47.5, 51.8 ms
 constexpr int kCoreMinDegree = 384;
-constexpr int kCoreSample = 64;      // in-degree sampled every 64th entry
+constexpr int kCoreSample = 128;     // in-degree sampled every 128th entry
 constexpr int kCoreBins = 2048;      // histogram: est < 64 exact, then 64 bins per octave
 constexpr int kCorePart = 2048;      // dense targets per work item (longer lists are cut)
 constexpr int kCoreThreads = 256;
